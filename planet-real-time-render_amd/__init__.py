@@ -1,0 +1,22 @@
+"""MI355X-native atmosphere hot path: LUT builds and per-pixel composite.
+
+Drop-in for the reference's `planet_renderer.atmosphere` package
+(src/planet_renderer/atmosphere/__init__.py): `AtmosphereConfig`,
+`AtmosphereDiagnosticView`, `AtmosphereRenderer`, plus `LightingState`
+(src/planet_renderer/lighting.py).  Compute runs in libatmo.so (HIP,
+gfx950); this package is its host side.
+"""
+
+from .config import AtmosphereConfig
+from .diagnostics import AtmosphereDiagnosticView
+from .lighting import LightingState, StaticLightingProvider
+from .renderer import AtmosphereRenderer, DeviceField
+
+__all__ = [
+    "AtmosphereConfig",
+    "AtmosphereDiagnosticView",
+    "AtmosphereRenderer",
+    "DeviceField",
+    "LightingState",
+    "StaticLightingProvider",
+]

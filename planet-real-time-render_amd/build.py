@@ -1,0 +1,80 @@
+"""Build recipe of libatmo.so: hipcc, gfx950, in-tree.
+
+Flags:
+  -ffp-contract=off   no FMA contraction: every float32 operation rounds
+                      as written, matching the operation order of the
+                      reference's formulas;
+  -Xclang -target-feature -Xclang -packed-fp32-ops
+                      no packed-FP32 instructions (v_pk_*_f32).
+HIP's default IEEE division and square root are kept.
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+PACKAGE_DIR = Path(__file__).resolve().parent
+CSRC = PACKAGE_DIR / "csrc"
+# library -> sources: libatmo.so is the product; libatmo_probe.so holds the
+# instruction-throughput probe the benchmark uses for its roofline peak.
+TARGETS = {
+    PACKAGE_DIR / "libatmo.so": [CSRC / "atmo_kernels.hip"],
+    PACKAGE_DIR / "libatmo_probe.so": [CSRC / "probe.hip"],
+}
+ARCH = "gfx950"
+
+HIPCC_FLAGS = [
+    "-O3",
+    f"--offload-arch={ARCH}",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-ffp-contract=off",
+    "-Xclang",
+    "-target-feature",
+    "-Xclang",
+    "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-function",
+]
+
+
+def hipcc() -> str:
+    for candidate in (os.environ.get("HIPCC"), shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if candidate and Path(candidate).exists():
+            return candidate
+    raise RuntimeError("hipcc not found (ROCm is required to build libatmo.so)")
+
+
+def _compile(output: Path, sources: list[Path], verbose: bool) -> Path:
+    fd, tmp = tempfile.mkstemp(prefix=output.stem + ".", suffix=".so.tmp", dir=str(PACKAGE_DIR))
+    os.close(fd)
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(CSRC), "-o", tmp, *map(str, sources)]
+    try:
+        result = subprocess.run(cmd, capture_output=True, text=True)
+        noise = "packed-fp32-ops"  # host pass does not know the device feature
+        log = "\n".join(line for line in (result.stdout + result.stderr).splitlines()
+                        if noise not in line)
+        if result.returncode != 0:
+            raise RuntimeError(f"hipcc failed ({result.returncode}) for {output.name}:\n{log}")
+        if verbose and log.strip():
+            print(log, file=sys.stderr)
+        os.replace(tmp, output)
+    finally:
+        if os.path.exists(tmp):
+            os.unlink(tmp)
+    return output
+
+
+def build(verbose: bool = False) -> list[Path]:
+    """Compile every HIP library (always recompiles)."""
+    return [_compile(out, srcs, verbose) for out, srcs in TARGETS.items()]
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

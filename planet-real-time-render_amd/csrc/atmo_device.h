@@ -1,0 +1,705 @@
+// atmo_device.h -- device-side atmosphere model for gfx950.
+//
+// Each __device__ function computes what the @ti.func of the same name in
+// the reference's src/planet_renderer/atmosphere/renderer.py computes (line
+// ranges cited per function), in float32 with the reference's operation
+// order.  The library is compiled with -ffp-contract=off and IEEE division
+// and square root, so the only source of difference from an IEEE float32
+// evaluation of the reference formulas is the ulp-level error of the
+// device exp/acos/cos/sin/pow/atan2.
+//
+// Frame- and LUT-sized arrays use the reference's Taichi field layout:
+// (X, Y[, Z], 3) float32, element [x, y] at (x * Y + y) * 3.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace atmo {
+
+// Per-state constants.  Values the reference reads through `ti.static` are
+// rounded once from the configuration's float64 value; products of Python
+// constants (16*pi, 1+g^2, 2g, 3(1-g^2), 8*pi*(2+g^2)) are folded in
+// float64 first, as Python folds them before Taichi sees them
+// (renderer.py:792-804).
+struct Consts {
+    float pi;
+    float pi16;
+    float large;
+    float top;
+    float beta_r[3], beta_ms[3], beta_me[3], beta_a[3], albedo[3];
+    float h_r, h_m, peak, half_width;
+    float mie_a, mie_b, mie_c, mie_d;
+    int tw, th, mw, mh, sw, sh, aw, ah, ad;
+    int t_steps, ms_dirs, ms_steps, sv_steps, ap_spp, hz_steps, term_sub;
+};
+
+struct f3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 operator/(f3 a, f3 b) { return f3{a.x / b.x, a.y / b.y, a.z / b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ float norm(f3 a) { return sqrtf(dot(a, a)); }
+// Taichi: normalized(v) = (1 / norm(v)) * v.
+__device__ __forceinline__ f3 normalized(f3 a) {
+    float inv = 1.0f / norm(a);
+    return f3{inv * a.x, inv * a.y, inv * a.z};
+}
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(hi, fmaxf(lo, x)); }
+__device__ __forceinline__ f3 maxs(f3 a, float s) { return f3{fmaxf(a.x, s), fmaxf(a.y, s), fmaxf(a.z, s)}; }
+__device__ __forceinline__ f3 mins(f3 a, float s) { return f3{fminf(a.x, s), fminf(a.y, s), fminf(a.z, s)}; }
+__device__ __forceinline__ f3 clamp3(f3 a, float lo, float hi) {
+    return f3{clampf(a.x, lo, hi), clampf(a.y, lo, hi), clampf(a.z, lo, hi)};
+}
+__device__ __forceinline__ f3 exp3(f3 a) { return f3{expf(a.x), expf(a.y), expf(a.z)}; }
+__device__ __forceinline__ f3 arr3(const float* a) { return f3{a[0], a[1], a[2]}; }
+__device__ __forceinline__ f3 ld3(const float* __restrict__ base, long i) {
+    return f3{base[i * 3 + 0], base[i * 3 + 1], base[i * 3 + 2]};
+}
+__device__ __forceinline__ void st3(float* __restrict__ base, long i, f3 v) {
+    base[i * 3 + 0] = v.x;
+    base[i * 3 + 1] = v.y;
+    base[i * 3 + 2] = v.z;
+}
+
+// ---------------------------------------------------------------- media
+
+// renderer.py:168-192
+__device__ __forceinline__ f3 density(const Consts& k, float h) {
+    float rayleigh = 0.0f, mie = 0.0f, absorption = 0.0f;
+    if (0.0f <= h && h <= k.top) {
+        rayleigh = expf(-h / k.h_r);
+        mie = expf(-h / k.h_m);
+        absorption = fmaxf(1.0f - fabsf(h - k.peak) / k.half_width, 0.0f);
+    }
+    return f3{rayleigh, mie, absorption};
+}
+
+// renderer.py:194-201, from a density triple.
+__device__ __forceinline__ f3 extinction_of(const Consts& k, f3 d) {
+    return (arr3(k.beta_r) * d.x + arr3(k.beta_me) * d.y) + arr3(k.beta_a) * d.z;
+}
+
+// renderer.py:203-209, from a density triple.
+__device__ __forceinline__ f3 scattering_of(const Consts& k, f3 d) {
+    return arr3(k.beta_r) * d.x + arr3(k.beta_ms) * d.y;
+}
+
+// ------------------------------------------------------------- geometry
+
+// renderer.py:211-226
+__device__ __forceinline__ void sphere_roots(const Consts& k, f3 origin, f3 ray, float radius,
+                                             float& near, float& far, bool& hit) {
+    float projected = dot(origin, ray);
+    float perpendicular = norm(origin - ray * projected);
+    float clearance = radius - perpendicular;
+    hit = clearance >= 0.0f;
+    near = k.large;
+    far = -k.large;
+    if (hit) {
+        float disc = fmaxf(clearance, 0.0f) * (radius + perpendicular);
+        float root = sqrtf(fmaxf(disc, 0.0f));
+        near = -projected - root;
+        far = -projected + root;
+    }
+}
+
+// renderer.py:228-235
+__device__ __forceinline__ float shell_rho(float altitude, float bottom) {
+    float s = fmaxf(altitude, 0.0f);
+    return sqrtf(s * (2.0f * bottom + s));
+}
+
+// renderer.py:237-262
+__device__ __forceinline__ float altitude_on_ray(float h0, float c, float t, float bottom) {
+    float a = fmaxf(h0, 0.0f);
+    float r0 = bottom + a;
+    float delta = a * (2.0f * bottom + a) + 2.0f * r0 * c * t + t * t;
+    float len = sqrtf(fmaxf(bottom * bottom + delta, 0.0f));
+    return delta / fmaxf(len + bottom, 1.0f);
+}
+
+// renderer.py:264-301
+__device__ __forceinline__ void shell_roots(const Consts& k, float h0, float c, float shell,
+                                            float bottom, float& near, float& far, bool& hit) {
+    float r0 = bottom + h0;
+    float hl = r0 * c;
+    float constant = (h0 - shell) * (2.0f * bottom + h0 + shell);
+    float disc = hl * hl - constant;
+    hit = disc >= 0.0f;
+    near = k.large;
+    far = -k.large;
+    if (hit) {
+        float root = sqrtf(fmaxf(disc, 0.0f));
+        float sign = hl < 0.0f ? -1.0f : 1.0f;
+        float q = -hl - sign * root;
+        float root0 = -hl - root;
+        float root1 = -hl + root;
+        if (fabsf(q) > 1.0e-12f) {
+            root0 = q;
+            root1 = constant / q;
+        }
+        near = fminf(root0, root1);
+        far = fmaxf(root0, root1);
+    }
+}
+
+// renderer.py:303-340
+__device__ __forceinline__ void segment_from_altitude(const Consts& k, float h, float c,
+                                                      float bottom, float& start, float& end,
+                                                      bool& valid, bool& ground) {
+    float on, of, gn, gf;
+    bool oh, gh;
+    shell_roots(k, h, c, k.top, bottom, on, of, oh);
+    start = fmaxf(on, 0.0f);
+    end = of;
+    valid = oh && end > start;
+    ground = false;
+    shell_roots(k, h, c, 0.0f, bottom, gn, gf, gh);
+    if (valid && gh) {
+        float gd = k.large;
+        const float eps = 1.0e-3f;
+        if (gn > start + eps)
+            gd = gn;
+        else if (gf > start + eps)
+            gd = gf;
+        if (gd < end) {
+            end = gd;
+            ground = true;
+        }
+    }
+}
+
+// renderer.py:342-374
+__device__ __forceinline__ void segment_vec(const Consts& k, f3 origin, f3 ray, float bottom,
+                                            float& start, float& end, bool& valid, bool& ground) {
+    float top_radius = bottom + k.top;
+    float on, of, gn, gf;
+    bool oh, gh;
+    sphere_roots(k, origin, ray, top_radius, on, of, oh);
+    start = fmaxf(on, 0.0f);
+    end = of;
+    valid = oh && end > start;
+    ground = false;
+    sphere_roots(k, origin, ray, bottom, gn, gf, gh);
+    if (valid && gh) {
+        float gd = k.large;
+        float eps = fmaxf(0.001f, bottom * 1.0e-8f);
+        if (gn > start + eps)
+            gd = gn;
+        else if (gf > start + eps)
+            gd = gf;
+        if (gd < end) {
+            end = gd;
+            ground = true;
+        }
+    }
+}
+
+// renderer.py:376-395
+__device__ __forceinline__ void aerial_prefix(const Consts& k, float h, float c, float bottom,
+                                              float& start, float& end, bool& valid) {
+    bool ground;
+    segment_from_altitude(k, h, c, bottom, start, end, valid, ground);
+    if (valid && !ground) {
+        float closest = -(bottom + h) * c;
+        if (closest > start && closest < end) end = closest;
+    }
+    valid = valid && end > start;
+}
+
+// renderer.py:397-408
+__device__ __forceinline__ float distance_to_top(const Consts& k, float h, float c, float bottom) {
+    float radius = bottom + fmaxf(h, 0.0f);
+    float shell_delta = (k.top - h) * (2.0f * bottom + k.top + h);
+    float disc = radius * radius * c * c + shell_delta;
+    return fmaxf(-radius * c + sqrtf(fmaxf(disc, 0.0f)), 0.0f);
+}
+
+// renderer.py:410-432
+__device__ __forceinline__ void transmittance_uv_to_ray(const Consts& k, float u, float v,
+                                                        float bottom, float& altitude,
+                                                        float& cosine, float& distance) {
+    float top = k.top;
+    float sh = shell_rho(top, bottom);
+    float rho = sh * clampf(v, 0.0f, 1.0f);
+    float rl = sqrtf(rho * rho + bottom * bottom);
+    float alt = rho * rho / fmaxf(rl + bottom, 1.0f);
+    float radius = bottom + alt;
+    float dmin = top - alt;
+    float dmax = rho + sh;
+    float d = dmin + clampf(u, 0.0f, 1.0f) * (dmax - dmin);
+    float cs = 1.0f;
+    if (d > 1.0e-6f) cs = ((top - alt) * (2.0f * bottom + top + alt) - d * d) / (2.0f * radius * d);
+    altitude = alt;
+    cosine = clampf(cs, -1.0f, 1.0f);
+    distance = d;
+}
+
+// renderer.py:434-452
+__device__ __forceinline__ void transmittance_ray_to_uv(const Consts& k, float h, float c,
+                                                        float bottom, float& u, float& v) {
+    float top = k.top;
+    h = clampf(h, 0.0f, top);
+    float sh = shell_rho(top, bottom);
+    float rho = shell_rho(h, bottom);
+    float d = distance_to_top(k, h, c, bottom);
+    float dmin = top - h;
+    float dmax = rho + sh;
+    u = clampf((d - dmin) / fmaxf(dmax - dmin, 1.0e-6f), 0.0f, 1.0f);
+    v = clampf(rho / fmaxf(sh, 1.0e-6f), 0.0f, 1.0f);
+}
+
+// Warped interval boundaries (renderer.py:454-480).  The closest-approach
+// classification depends only on the interval, not on the fraction, so it is
+// evaluated once per interval; boundary() then reproduces the reference's
+// per-fraction arithmetic exactly.
+struct Warp {
+    float start, end, closest;
+    int mode;  // 0: near start, 1: near end, 2: interior
+};
+
+__device__ __forceinline__ Warp make_warp(float neg_origin_dot_ray, float start, float end) {
+    Warp w;
+    w.start = start;
+    w.end = end;
+    w.closest = clampf(neg_origin_dot_ray, start, end);
+    float eps = fmaxf((end - start) * 1.0e-6f, 1.0e-5f);
+    if (w.closest <= start + eps)
+        w.mode = 0;
+    else if (w.closest >= end - eps)
+        w.mode = 1;
+    else
+        w.mode = 2;
+    return w;
+}
+
+__device__ __forceinline__ float boundary(const Warp& w, float f) {
+    if (w.mode == 0) return w.start + (w.end - w.start) * f * f;
+    if (w.mode == 1) {
+        float inv = 1.0f - f;
+        return w.end - (w.end - w.start) * inv * inv;
+    }
+    if (f < 0.5f) {
+        float local = f * 2.0f;
+        float inv = 1.0f - local;
+        return w.closest - (w.closest - w.start) * inv * inv;
+    }
+    float local = f * 2.0f - 1.0f;
+    return w.closest + (w.end - w.closest) * local * local;
+}
+
+// ------------------------------------------------------ LUT coordinates
+
+// renderer.py:482-490
+__device__ __forceinline__ float sky_horizon_zenith(const Consts& k, float h, float bottom) {
+    float a = fmaxf(h, 0.0f);
+    float radius = bottom + a;
+    float tc = shell_rho(a, bottom) / fmaxf(radius, 1.0f);
+    return k.pi - acosf(clampf(tc, 0.0f, 1.0f));
+}
+
+// renderer.py:492-504
+__device__ __forceinline__ float sky_v_to_zenith(const Consts& k, float v, float horizon) {
+    if (v < 0.5f) {
+        float inv = 1.0f - 2.0f * v;
+        return horizon * (1.0f - inv * inv);
+    }
+    float local = 2.0f * v - 1.0f;
+    return horizon + (k.pi - horizon) * local * local;
+}
+
+// renderer.py:506-520
+__device__ __forceinline__ float sky_zenith_to_v(const Consts& k, float zenith, float horizon) {
+    if (zenith < horizon) {
+        float n = clampf(zenith / fmaxf(horizon, 1.0e-6f), 0.0f, 1.0f);
+        return 0.5f * (1.0f - sqrtf(fmaxf(1.0f - n, 0.0f)));
+    }
+    float n = clampf((zenith - horizon) / fmaxf(k.pi - horizon, 1.0e-6f), 0.0f, 1.0f);
+    return 0.5f + 0.5f * sqrtf(n);
+}
+
+// renderer.py:522-542
+__device__ __forceinline__ float sun_u_to_cosine(float u, float h, float bottom) {
+    float radius = bottom + fmaxf(h, 0.0f);
+    float hc = -shell_rho(h, bottom) / fmaxf(radius, 1.0f);
+    float c;
+    if (u < 0.5f) {
+        float inv = 1.0f - 2.0f * u;
+        c = hc - (hc + 1.0f) * inv * inv;
+    } else {
+        float local = 2.0f * u - 1.0f;
+        c = hc + (1.0f - hc) * local * local;
+    }
+    return clampf(c, -1.0f, 1.0f);
+}
+
+// renderer.py:544-568
+__device__ __forceinline__ float sun_cosine_to_u(float c, float h, float bottom) {
+    float radius = bottom + fmaxf(h, 0.0f);
+    float hc = -shell_rho(h, bottom) / fmaxf(radius, 1.0f);
+    if (c < hc) {
+        float n = clampf((hc - c) / fmaxf(hc + 1.0f, 1.0e-6f), 0.0f, 1.0f);
+        return 0.5f * (1.0f - sqrtf(n));
+    }
+    float n = clampf((c - hc) / fmaxf(1.0f - hc, 1.0e-6f), 0.0f, 1.0f);
+    return 0.5f + 0.5f * sqrtf(n);
+}
+
+// ------------------------------------------------------------- sampling
+
+// Half-texel bilinear fetch (renderer.py:618-640, 655-683, 1503-1525).
+__device__ __forceinline__ f3 bilinear(const float* __restrict__ lut, int w, int hgt, float u,
+                                       float v) {
+    float x = clampf(u * (float)w - 0.5f, 0.0f, (float)(w - 1));
+    float y = clampf(v * (float)hgt - 0.5f, 0.0f, (float)(hgt - 1));
+    int x0 = (int)floorf(x);
+    int y0 = (int)floorf(y);
+    int x1 = min(x0 + 1, w - 1);
+    int y1 = min(y0 + 1, hgt - 1);
+    float fx = x - (float)x0;
+    float fy = y - (float)y0;
+    f3 low = ld3(lut, (long)x0 * hgt + y0) * (1.0f - fx) + ld3(lut, (long)x1 * hgt + y0) * fx;
+    f3 high = ld3(lut, (long)x0 * hgt + y1) * (1.0f - fx) + ld3(lut, (long)x1 * hgt + y1) * fx;
+    return low * (1.0f - fy) + high * fy;
+}
+
+// renderer.py:609-640
+__device__ __forceinline__ f3 sample_transmittance(const Consts& k, const float* __restrict__ tlut,
+                                                   float h, float c, float bottom) {
+    float u, v;
+    transmittance_ray_to_uv(k, fmaxf(h, 0.0f), c, bottom, u, v);
+    return bilinear(tlut, k.tw, k.th, u, v);
+}
+
+// renderer.py:642-683
+__device__ __forceinline__ f3 sample_multi_scattering(const Consts& k,
+                                                      const float* __restrict__ mslut, float h,
+                                                      float sun_cosine, float bottom) {
+    float af = clampf(h / k.top, 0.0f, 1.0f);
+    float u = sun_cosine_to_u(sun_cosine, fmaxf(h, 0.0f), bottom);
+    float v = sqrtf(af);
+    return bilinear(mslut, k.mw, k.mh, u, v);
+}
+
+// ------------------------------------------------------- sun and phases
+
+// renderer.py:685-726
+__device__ __forceinline__ void solar_shadow_roots(f3 origin, f3 ray, f3 sun, float bottom,
+                                                   float& r0, float& r1, bool& v0, bool& v1) {
+    f3 op = origin - sun * dot(origin, sun);
+    f3 rp = ray - sun * dot(ray, sun);
+    float quadratic = dot(rp, rp);
+    float hl = dot(op, rp);
+    float pl = norm(op);
+    float constant = (pl - bottom) * (pl + bottom);
+    r0 = 0.0f;
+    r1 = 0.0f;
+    v0 = false;
+    v1 = false;
+    float disc = hl * hl - quadratic * constant;
+    if (quadratic > 1.0e-8f && disc >= 0.0f) {
+        float sr = sqrtf(fmaxf(disc, 0.0f));
+        r0 = (-hl - sr) / quadratic;
+        r1 = (-hl + sr) / quadratic;
+        f3 p0 = origin + ray * r0;
+        f3 p1 = origin + ray * r1;
+        v0 = dot(p0, sun) < 0.0f;
+        v1 = dot(p1, sun) < 0.0f;
+    }
+}
+
+// The quantities _solar_disk_visibility and _sample_solar_transmittance
+// both derive from (altitude, sun cosine) (renderer.py:728-789), computed
+// once and shared.
+struct SunGeom {
+    float visibility;
+    float horizon_zenith;
+    float sun_zenith;
+};
+
+__device__ __forceinline__ SunGeom sun_geometry(const Consts& k, float h, float sun_cosine,
+                                                float bottom, float angular_radius) {
+    SunGeom g;
+    float radius = bottom + fmaxf(h, 0.0f);
+    float hc = -shell_rho(h, bottom) / fmaxf(radius, 1.0f);
+    g.horizon_zenith = acosf(clampf(hc, -1.0f, 1.0f));
+    g.sun_zenith = acosf(clampf(sun_cosine, -1.0f, 1.0f));
+    float sep = g.horizon_zenith - g.sun_zenith;
+    float n = clampf(sep / fmaxf(angular_radius, 1.0e-6f), -1.0f, 1.0f);
+    g.visibility = (acosf(-n) + n * sqrtf(fmaxf(1.0f - n * n, 0.0f))) / k.pi;
+    return g;
+}
+
+// renderer.py:756-789, given the shared geometry.
+__device__ __forceinline__ f3 solar_transmittance(const Consts& k, const float* __restrict__ tlut,
+                                                  const SunGeom& g, float h, float bottom,
+                                                  float angular_radius) {
+    float rep = fminf(g.sun_zenith, g.horizon_zenith - angular_radius * 0.25f);
+    float rc = cosf(fmaxf(rep, 0.0f));
+    return sample_transmittance(k, tlut, h, rc, bottom) * g.visibility;
+}
+
+// renderer.py:791-793
+__device__ __forceinline__ float rayleigh_phase(const Consts& k, float c) {
+    return 3.0f * (1.0f + c * c) / k.pi16;
+}
+
+// renderer.py:795-804
+__device__ __forceinline__ float mie_phase(const Consts& k, float c) {
+    float den = fmaxf(k.mie_a - k.mie_b * c, 1.0e-8f);
+    return k.mie_c * (1.0f + c * c) / (k.mie_d * powf(den, 1.5f));
+}
+
+// renderer.py:806-816
+__device__ __forceinline__ void segment_integral(f3 ext, float step, f3& trans, f3& integ) {
+    trans = exp3(-mins(ext * step, 80.0f));
+    integ.x = ext.x > 1.0e-12f ? (1.0f - trans.x) / ext.x : step;
+    integ.y = ext.y > 1.0e-12f ? (1.0f - trans.y) / ext.y : step;
+    integ.z = ext.z > 1.0e-12f ? (1.0f - trans.z) / ext.z : step;
+}
+
+// Phase-weighted scattering coefficients of one ray: the view/sun cosine is
+// constant along a ray, so P_R and P_M are evaluated once per ray
+// (renderer.py:840-848).
+struct RayPhase {
+    float pr, pm;  // Rayleigh and Cornette-Shanks phase at the ray/sun cosine
+};
+
+__device__ __forceinline__ RayPhase ray_phase(const Consts& k, f3 ray, f3 sun) {
+    RayPhase p;
+    float c = clampf(dot(ray, sun), -1.0f, 1.0f);
+    p.pr = rayleigh_phase(k, c);
+    p.pm = mie_phase(k, c);
+    return p;
+}
+
+// renderer.py:818-856.  `point_radius` is |point|; the caller's density
+// triple `d` and altitude are those of `point`.
+template <bool kMultiple>
+__device__ __forceinline__ f3 scattering_source(const Consts& k, const float* __restrict__ tlut,
+                                                const float* __restrict__ mslut, f3 point,
+                                                float h, f3 d, const RayPhase& ph, f3 sun,
+                                                f3 irradiance, float bottom, float ar) {
+    float radius = norm(point);
+    f3 radial = point / fmaxf(radius, 1.0f);
+    float sun_cosine = dot(radial, sun);
+    SunGeom g = sun_geometry(k, h, sun_cosine, bottom, ar);
+    f3 st = solar_transmittance(k, tlut, g, h, bottom, ar);
+    f3 phase_term = (arr3(k.beta_r) * d.x) * ph.pr + (arr3(k.beta_ms) * d.y) * ph.pm;
+    f3 source = (irradiance * st) * phase_term;
+    if (kMultiple) {
+        f3 ms = sample_multi_scattering(k, mslut, h, sun_cosine, bottom);
+        source = source + scattering_of(k, d) * ms;
+    }
+    return source;
+}
+
+}  // namespace atmo
+
+namespace atmo {
+
+// One constant-source segment (renderer.py:858-899): returns L and T.
+// `oc` is origin_radial . ray for the ray's origin on the local radial axis.
+__device__ __forceinline__ void scattering_segment(const Consts& k, const float* __restrict__ tlut,
+                                                   const float* __restrict__ mslut, f3 origin,
+                                                   float h0, float oc, f3 ray, float start,
+                                                   float end, const RayPhase& ph, f3 sun,
+                                                   f3 irradiance, float bottom, float ar, f3& L,
+                                                   f3& T) {
+    float step = fmaxf(end - start, 0.0f);
+    float distance = (start + end) * 0.5f;
+    f3 point = origin + ray * distance;
+    float alt = altitude_on_ray(h0, oc, distance, bottom);
+    f3 d = density(k, alt);
+    f3 integ;
+    segment_integral(extinction_of(k, d), step, T, integ);
+    L = scattering_source<true>(k, tlut, mslut, point, alt, d, ph, sun, irradiance, bottom, ar) *
+        integ;
+}
+
+// The bounded direct limb integral of a surface pixel
+// (_integrate_surface_scattering, renderer.py:1667-1755, with the origin of
+// _surface_aerial_scattering, renderer.py:1757-1786): base intervals with
+// continuous terminator refinement (renderer.py:901-1032).  The refinement
+// weight and the coarse segment of an interval share one midpoint, so the
+// sun geometry at that point is computed once for both.
+__device__ __forceinline__ f3 surface_inscatter(const Consts& k, const float* __restrict__ tlut,
+                                                const float* __restrict__ mslut, float h0, f3 ray,
+                                                float surface_distance, f3 sun, f3 irradiance,
+                                                float bottom, float ar) {
+    f3 radiance = mk(0.0f, 0.0f, 0.0f);
+    f3 view_t = mk(1.0f, 1.0f, 1.0f);
+    float on, of;
+    bool oh;
+    shell_roots(k, h0, ray.y, k.top, bottom, on, of, oh);
+    float start = fmaxf(on, 0.0f);
+    float end = fminf(surface_distance, of);
+    if (!(oh && end > start)) return radiance;
+
+    const f3 origin = mk(0.0f, bottom + h0, 0.0f);
+    float r0, r1;
+    bool v0, v1;
+    solar_shadow_roots(origin, ray, sun, bottom, r0, r1, v0, v1);
+    const Warp w = make_warp(-dot(origin, ray), start, end);
+    const float origin_radius = fmaxf(norm(origin), 1.0f);
+    const float oc = dot(origin / origin_radius, ray);
+    const RayPhase ph = ray_phase(k, ray, sun);
+    const int n = k.hz_steps;
+    float d0 = boundary(w, 0.0f / (float)n);
+    for (int i = 0; i < n; ++i) {
+        float d1 = boundary(w, (float)(i + 1) / (float)n);
+        // _terminator_refinement_weight
+        float mid = (d0 + d1) * 0.5f;
+        float width = fmaxf(d1 - d0, 1.0e-4f);
+        float closest = 1.0e30f;
+        if (v0) closest = fminf(closest, fabsf(mid - r0));
+        if (v1) closest = fminf(closest, fabsf(mid - r1));
+        float prox = clampf(2.0f - closest / width, 0.0f, 1.0f);
+        prox = prox * prox * (3.0f - 2.0f * prox);
+        f3 point = origin + ray * mid;
+        float radius = fmaxf(norm(point), 1.0f);
+        f3 radial = point / radius;
+        float alt = altitude_on_ray(h0, oc, mid, bottom);
+        float sun_cosine = dot(radial, sun);
+        SunGeom g = sun_geometry(k, alt, sun_cosine, bottom, ar);
+        float weight = fmaxf(prox, 4.0f * g.visibility * (1.0f - g.visibility));
+        // coarse _scattering_segment at the same midpoint
+        f3 cl, ct;
+        {
+            float step = fmaxf(d1 - d0, 0.0f);
+            f3 d = density(k, alt);
+            f3 integ;
+            segment_integral(extinction_of(k, d), step, ct, integ);
+            f3 st = solar_transmittance(k, tlut, g, alt, bottom, ar);
+            f3 phase_term = (arr3(k.beta_r) * d.x) * ph.pr + (arr3(k.beta_ms) * d.y) * ph.pm;
+            f3 src = (irradiance * st) * phase_term;
+            f3 ms = sample_multi_scattering(k, mslut, alt, sun_cosine, bottom);
+            src = src + scattering_of(k, d) * ms;
+            cl = src * integ;
+        }
+        f3 il = cl, it = ct;
+        if (weight > 0.0f) {
+            f3 fl = mk(0.0f, 0.0f, 0.0f);
+            f3 ft = mk(1.0f, 1.0f, 1.0f);
+            const int m = k.term_sub;
+            for (int j = 0; j < m; ++j) {
+                float f0 = (float)j / (float)m;
+                float f1 = (float)(j + 1) / (float)m;
+                float cs = d0 + (d1 - d0) * f0;
+                float ce = d0 + (d1 - d0) * f1;
+                f3 sl, stt;
+                scattering_segment(k, tlut, mslut, origin, h0, oc, ray, cs, ce, ph, sun,
+                                   irradiance, bottom, ar, sl, stt);
+                fl = fl + ft * sl;
+                ft = ft * stt;
+            }
+            il = cl * (1.0f - weight) + fl * weight;
+            it = ct * (1.0f - weight) + ft * weight;
+        }
+        radiance = radiance + view_t * il;
+        view_t = view_t * it;
+        d0 = d1;
+    }
+    return maxs(radiance, 0.0f);
+}
+
+// Camera-to-surface transmittance from two unoccluded T-LUT samples along
+// the reversed ray (_surface_segment_transmittance, renderer.py:1603-1665).
+__device__ __forceinline__ f3 surface_transmittance(const Consts& k,
+                                                    const float* __restrict__ tlut, float h0,
+                                                    f3 ray, float surface_distance,
+                                                    float bottom) {
+    f3 t = mk(1.0f, 1.0f, 1.0f);
+    float on, of;
+    bool oh;
+    shell_roots(k, h0, ray.y, k.top, bottom, on, of, oh);
+    float start = fmaxf(on, 0.0f);
+    float end = fminf(surface_distance, of);
+    if (oh && end > start) {
+        const f3 origin = mk(0.0f, bottom + h0, 0.0f);
+        f3 near_point = origin + ray * start;
+        f3 surface_point = origin + ray * end;
+        f3 reverse = -ray;
+        f3 near_radial = near_point / fmaxf(norm(near_point), 1.0f);
+        f3 surface_radial = surface_point / fmaxf(norm(surface_point), 1.0f);
+        float near_alt = altitude_on_ray(h0, ray.y, start, bottom);
+        float surface_alt = altitude_on_ray(h0, ray.y, end, bottom);
+        f3 s2t = sample_transmittance(k, tlut, surface_alt, dot(surface_radial, reverse), bottom);
+        f3 n2t = sample_transmittance(k, tlut, near_alt, dot(near_radial, reverse), bottom);
+        t = clamp3(s2t / maxs(n2t, 1.0e-6f), 0.0f, 1.0f);
+    }
+    return t;
+}
+
+// renderer.py:1788-1825
+__device__ __forceinline__ f3 camera_transmittance(const Consts& k,
+                                                   const float* __restrict__ tlut, float h,
+                                                   f3 ray, float bottom) {
+    float start, end;
+    bool valid, ground;
+    segment_from_altitude(k, h, ray.y, bottom, start, end, valid, ground);
+    f3 t = mk(1.0f, 1.0f, 1.0f);
+    if (valid) {
+        if (ground) {
+            t = mk(0.0f, 0.0f, 0.0f);
+        } else {
+            f3 origin = mk(0.0f, bottom + h, 0.0f);
+            f3 entry = origin + ray * start;
+            f3 radial = entry / fmaxf(norm(entry), 1.0f);
+            float ea = altitude_on_ray(h, ray.y, start, bottom);
+            t = sample_transmittance(k, tlut, ea, dot(radial, ray), bottom);
+        }
+    }
+    return t;
+}
+
+// renderer.py:1474-1525
+__device__ __forceinline__ f3 sample_sky_view(const Consts& k, const float* __restrict__ svlut,
+                                              f3 ray, f3 sun, float h, float bottom) {
+    float zenith = acosf(clampf(ray.y, -1.0f, 1.0f));
+    float vl = sqrtf(ray.x * ray.x + ray.z * ray.z);
+    float sl = sqrtf(sun.x * sun.x + sun.z * sun.z);
+    float rel = 0.0f;
+    if (vl > 1.0e-6f && sl > 1.0e-6f)
+        rel = acosf(clampf((ray.x * sun.x + ray.z * sun.z) / (vl * sl), -1.0f, 1.0f));
+    float u = rel / k.pi;
+    float v = sky_zenith_to_v(k, zenith, sky_horizon_zenith(k, h, bottom));
+    return bilinear(svlut, k.sw, k.sh, u, v);
+}
+
+// renderer.py:1527-1601
+__device__ __forceinline__ f3 sample_aerial(const Consts& k, const float* __restrict__ ap,
+                                            float su, float sv, float distance, float h, f3 ray,
+                                            float bottom) {
+    float start, end;
+    bool valid;
+    aerial_prefix(k, h, ray.y, bottom, start, end, valid);
+    float df = 0.0f;
+    if (valid) df = clampf((distance - start) / fmaxf(end - start, 1.0f), 0.0f, 1.0f);
+    const int aw = k.aw, ah = k.ah, ad = k.ad;
+    float x = clampf(su * (float)aw - 0.5f, 0.0f, (float)(aw - 1));
+    float y = clampf(sv * (float)ah - 0.5f, 0.0f, (float)(ah - 1));
+    float z = clampf(df, 0.0f, 1.0f) * (float)(ad - 1);
+    int x0 = (int)floorf(x), y0 = (int)floorf(y), z0 = (int)floorf(z);
+    int x1 = min(x0 + 1, aw - 1), y1 = min(y0 + 1, ah - 1), z1 = min(z0 + 1, ad - 1);
+    float fx = x - (float)x0, fy = y - (float)y0, fz = z - (float)z0;
+    auto at = [&](int X, int Y, int Z) { return ld3(ap, ((long)X * ah + Y) * ad + Z); };
+    f3 a = at(x0, y0, z0) * (1.0f - fx) + at(x1, y0, z0) * fx;
+    f3 b = at(x0, y1, z0) * (1.0f - fx) + at(x1, y1, z0) * fx;
+    f3 c = at(x0, y0, z1) * (1.0f - fx) + at(x1, y0, z1) * fx;
+    f3 e = at(x0, y1, z1) * (1.0f - fx) + at(x1, y1, z1) * fx;
+    f3 low = a * (1.0f - fy) + b * fy;
+    f3 high = c * (1.0f - fy) + e * fy;
+    return low * (1.0f - fz) + high * fz;
+}
+
+}  // namespace atmo
