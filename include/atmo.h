@@ -189,6 +189,89 @@ int32_t atmo_composite(
     float tangent_half_fov, float star_contrast_start,
     float star_contrast_end, int32_t diagnostic_view, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Learned surrogate of the composite's surface branch.  New in this library:
+ * the reference has no learned path.  The surrogate replaces, per surface
+ * pixel, the direct limb integral (_integrate_surface_scattering,
+ * renderer.py:1667-1786) and the two-sample segment transmittance
+ * (_surface_segment_transmittance, renderer.py:1603-1665) with a
+ * 5 -> 64 -> 64 -> 64 -> 6 SiLU MLP; out-of-domain pixels keep the plain
+ * integrator.
+ * --------------------------------------------------------------------- */
+
+/* Parameter count: W1 64x5, b1, W2 64x64, b2, W3 64x64, b3, W4 6x64, b4. */
+#define ATMO_MLP_PARAMS 9094
+
+/* Input normalisation, training domain and output decoding.  Inputs are
+ * (camera altitude h, view-zenith cosine, sun-zenith cosine, relative
+ * azimuth phi, segment length d) mapped to [-1, 1] (h / altitude_max_m,
+ * phi / pi and log d over [distance_min_m, distance_max_m] linearly).
+ * Outputs y (6) decode as L_c = exp(y_c * out_scale_c + out_mean_c) -
+ * eps_inscatter (in-scatter per unit irradiance, c = 0..2) and
+ * T_c = exp(y_{3+c} * out_scale + out_mean) - eps_transmittance. */
+typedef struct AtmoSurrogateTransform {
+    float altitude_max_m;
+    float distance_min_m;
+    float distance_max_m;
+    float out_mean[6];
+    float out_scale[6];
+    float eps_inscatter;
+    float eps_transmittance;
+} AtmoSurrogateTransform;
+
+typedef struct AtmoMlp AtmoMlp;
+
+/* Device-resident network, Adam state and training scratch for batches of
+ * up to max_batch samples. */
+int32_t atmo_mlp_create(int32_t max_batch, AtmoMlp** out);
+int32_t atmo_mlp_destroy(AtmoMlp* mlp);
+const char* atmo_mlp_last_error(const AtmoMlp* mlp);
+/* Host copies of the fp32 master parameters (count = ATMO_MLP_PARAMS).
+ * set_params also resets the Adam moments and repacks the BF16 inference
+ * weights. */
+int32_t atmo_mlp_set_params(AtmoMlp* mlp, const float* host_params, int64_t count);
+int32_t atmo_mlp_get_params(AtmoMlp* mlp, float* host_params, int64_t count);
+int32_t atmo_mlp_set_transform(AtmoMlp* mlp, const AtmoSurrogateTransform* transform);
+/* Raw samples (n, 5) = (h, view cos, sun cos, phi, d) -> encoded inputs. */
+int32_t atmo_mlp_encode(AtmoMlp* mlp, const float* raw, int64_t n, float* x, void* stream);
+/* FP32 training forward pass: x (B, 5) encoded -> y (B, 6) raw outputs. */
+int32_t atmo_mlp_forward(AtmoMlp* mlp, const float* x, int32_t batch, float* y, void* stream);
+/* Gradient of the mean-square error over rows index[0..B) (index may be
+ * NULL for rows 0..B) of x (.., 5) and target (.., 6); grad_out (device,
+ * ATMO_MLP_PARAMS) and loss_out (device, 1 float) may be NULL. */
+int32_t atmo_mlp_gradients(AtmoMlp* mlp, const float* x, const float* target,
+                           const int32_t* index, int32_t batch, float* grad_out,
+                           float* loss_out, void* stream);
+/* One Adam step (lr, betas 0.9 / 0.999, eps 1e-8; step counts from 1). */
+int32_t atmo_mlp_train_step(AtmoMlp* mlp, const float* x, const float* target,
+                            const int32_t* index, int32_t batch, float lr, int32_t step,
+                            float* loss_out, void* stream);
+/* Repacks the current parameters into BF16 MFMA fragments for inference. */
+int32_t atmo_mlp_pack(AtmoMlp* mlp, void* stream);
+/* BF16 MFMA inference (FP32 accumulation) on encoded inputs x (n, 5). */
+int32_t atmo_mlp_infer(AtmoMlp* mlp, const float* x, int64_t n, float* y, void* stream);
+
+/* Training labels from the plain integrator: raw samples (n, 5) ->
+ * (n, 6) = in-scatter per unit irradiance (RGB) and transmittance (RGB) of
+ * the composite's surface branch, using the handle's current T and MS
+ * LUTs. */
+int32_t atmo_surrogate_labels(AtmoHandle* handle, const float* raw, int64_t n,
+                              float bottom_radius, float sun_angular_radius,
+                              float* labels, void* stream);
+
+/* The COMPOSITE view of atmo_composite with the surrogate in place of the
+ * surface branch's in-scatter and transmittance for in-domain pixels; same
+ * arguments as atmo_composite without diagnostic_view. */
+int32_t atmo_composite_surrogate(
+    AtmoHandle* handle, AtmoMlp* mlp, const float* scene_hdr,
+    const float* space_hdr, const int32_t* surface_id, const float* position_view,
+    float* hdr, float bottom_radius, float camera_altitude,
+    float sky_lut_camera_altitude, const float sun_local[3],
+    const float solar_irradiance[3], const float sun_disk_radiance[3],
+    float sun_angular_radius, const float right[3], const float view_up[3],
+    const float forward[3], float tangent_half_fov, float star_contrast_start,
+    float star_contrast_end, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

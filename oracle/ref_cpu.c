@@ -22,9 +22,17 @@
  *     rounded once (16*pi, 1+g^2, 2g, 3(1-g^2), 8*pi*(2+g^2)); see
  *     make_consts().
  *  3. Arithmetic is IEEE-754 round-to-nearest with no FMA contraction and no
- *     reassociation (-ffp-contract=off, no -ffast-math).  Taichi's own
- *     fast-math flags are NOT reproduced: parity with Taichi's exp/acos/pow
- *     code generation is unpinned by any reference test.
+ *     reassociation (-ffp-contract=off, no -ffast-math), and every
+ *     elementary function (exp, acos, cos, sin, pow, atan2) is correctly
+ *     rounded: evaluated in float64 and rounded once to float32 (the cr_*
+ *     helpers).  A libm's own float32 approximations are not used: glibc
+ *     2.35's acosf differs from the correctly rounded value on ~8 % of
+ *     arguments, and the reference's formulas amplify such last-bit
+ *     differences up to ~1e3-fold (the finite-disk visibility near the
+ *     horizon, (1 - exp(-x)) / sigma for small x, the sky-view horizon
+ *     coordinate).  Taichi's own fast-math code generation is NOT
+ *     reproduced: parity with Taichi's exp/acos/pow is unpinned by any
+ *     reference test.
  *  4. Vector sums run left to right: dot(a,b) = (ax*bx + ay*by) + az*bz;
  *     norm = sqrt(dot(a,a)); normalized(v) = v * (1 / norm(v)).
  *  5. clamp(x, lo, hi) = min(hi, max(lo, x)); max/min are fmax/fmin.
@@ -58,6 +66,23 @@ _Static_assert(sizeof(real) == sizeof(double), "f64 oracle needs REAL=double");
 #endif
 
 #define EXPORT __attribute__((visibility("default")))
+
+/* Correctly rounded elementary functions (assumption 3). */
+#ifdef ORACLE_F32
+static inline float cr_exp(float x) { return (float)exp((double)x); }
+static inline float cr_acos(float x) { return (float)acos((double)x); }
+static inline float cr_cos(float x) { return (float)cos((double)x); }
+static inline float cr_sin(float x) { return (float)sin((double)x); }
+static inline float cr_pow(float x, float y) { return (float)pow((double)x, (double)y); }
+static inline float cr_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
+#else
+#define cr_exp exp
+#define cr_acos acos
+#define cr_cos cos
+#define cr_sin sin
+#define cr_pow pow
+#define cr_atan2 atan2
+#endif
 
 /* ------------------------------------------------------------------ */
 /* Constants and configuration (renderer.py:12-13, 118-166).           */
@@ -145,7 +170,7 @@ static inline v3 vmins(v3 a, real s) { return V(fmin(a.x, s), fmin(a.y, s), fmin
 static inline v3 vclamp(v3 a, real lo, real hi) {
     return V(clampr(a.x, lo, hi), clampr(a.y, lo, hi), clampr(a.z, lo, hi));
 }
-static inline v3 vexp(v3 a) { return V(exp(a.x), exp(a.y), exp(a.z)); }
+static inline v3 vexp(v3 a) { return V(cr_exp(a.x), cr_exp(a.y), cr_exp(a.z)); }
 static inline v3 varr(const real* a) { return V(a[0], a[1], a[2]); }
 static inline v3 vload(const real* base, long idx) {
     return V(base[idx * 3 + 0], base[idx * 3 + 1], base[idx * 3 + 2]);
@@ -165,8 +190,8 @@ static inline v3 density(const K* k, real h) {
     int inside = 0.0 <= h && h <= k->top;
     real rayleigh = 0.0, mie = 0.0, absorption = 0.0;
     if (inside) {
-        rayleigh = exp(-h / k->h_r);
-        mie = exp(-h / k->h_m);
+        rayleigh = cr_exp(-h / k->h_r);
+        mie = cr_exp(-h / k->h_m);
         absorption = fmax(1.0 - fabs(h - k->peak) / k->half_width, 0.0);
     }
     return V(rayleigh, mie, absorption);
@@ -392,7 +417,7 @@ static inline real sky_horizon_zenith(const K* k, real h, real bottom) {
     real altitude = fmax(h, 0.0);
     real radius = bottom + altitude;
     real tangent_cosine = shell_rho(altitude, bottom) / fmax(radius, 1.0);
-    return k->pi - acos(clampr(tangent_cosine, 0.0, 1.0));
+    return k->pi - cr_acos(clampr(tangent_cosine, 0.0, 1.0));
 }
 
 /* renderer.py:492-504 */
@@ -549,11 +574,11 @@ static inline real solar_disk_visibility(const K* k, real h, real sun_cosine,
                                          real bottom, real angular_radius) {
     real radius = bottom + fmax(h, 0.0);
     real horizon_cosine = -shell_rho(h, bottom) / fmax(radius, 1.0);
-    real horizon_zenith = acos(clampr(horizon_cosine, -1.0, 1.0));
-    real sun_zenith = acos(clampr(sun_cosine, -1.0, 1.0));
+    real horizon_zenith = cr_acos(clampr(horizon_cosine, -1.0, 1.0));
+    real sun_zenith = cr_acos(clampr(sun_cosine, -1.0, 1.0));
     real separation = horizon_zenith - sun_zenith;
     real n = clampr(separation / fmax(angular_radius, 1.0e-6), -1.0, 1.0);
-    return (acos(-n) + n * sqrt(fmax(1.0 - n * n, 0.0))) / k->pi;
+    return (cr_acos(-n) + n * sqrt(fmax(1.0 - n * n, 0.0))) / k->pi;
 }
 
 /* renderer.py:756-789 */
@@ -563,10 +588,10 @@ static inline v3 sample_solar_transmittance(const K* k, const real* tlut,
     real visibility = solar_disk_visibility(k, h, sun_cosine, bottom, angular_radius);
     real radius = bottom + fmax(h, 0.0);
     real horizon_cosine = -shell_rho(h, bottom) / fmax(radius, 1.0);
-    real horizon_zenith = acos(clampr(horizon_cosine, -1.0, 1.0));
-    real sun_zenith = acos(clampr(sun_cosine, -1.0, 1.0));
+    real horizon_zenith = cr_acos(clampr(horizon_cosine, -1.0, 1.0));
+    real sun_zenith = cr_acos(clampr(sun_cosine, -1.0, 1.0));
     real representative = fmin(sun_zenith, horizon_zenith - angular_radius * 0.25);
-    real representative_cosine = cos(fmax(representative, 0.0));
+    real representative_cosine = cr_cos(fmax(representative, 0.0));
     return vs(sample_transmittance(k, tlut, h, representative_cosine, bottom),
               visibility);
 }
@@ -579,7 +604,7 @@ static inline real rayleigh_phase(const K* k, real c) {
 /* renderer.py:795-804 */
 static inline real mie_phase(const K* k, real c) {
     real denominator = fmax(k->mie_a - k->mie_b * c, 1.0e-8);
-    return k->mie_c * (1.0 + c * c) / (k->mie_d * pow(denominator, (real)1.5));
+    return k->mie_c * (1.0 + c * c) / (k->mie_d * cr_pow(denominator, (real)1.5));
 }
 
 /* renderer.py:806-816 */
@@ -807,13 +832,13 @@ static inline v3 camera_transmittance(const K* k, const real* tlut, real h,
 /* renderer.py:1474-1525 */
 static inline v3 sample_sky_view(const K* k, const real* svlut, v3 ray, v3 sun,
                                  real h, real bottom) {
-    real zenith = acos(clampr(ray.y, -1.0, 1.0));
+    real zenith = cr_acos(clampr(ray.y, -1.0, 1.0));
     real vhx = ray.x, vhz = ray.z, shx = sun.x, shz = sun.z;
     real view_length = sqrt(vhx * vhx + vhz * vhz);
     real sun_length = sqrt(shx * shx + shz * shz);
     real relative_azimuth = 0.0;
     if (view_length > 1.0e-6 && sun_length > 1.0e-6)
-        relative_azimuth = acos(clampr((vhx * shx + vhz * shz)
+        relative_azimuth = cr_acos(clampr((vhx * shx + vhz * shz)
                                            / (view_length * sun_length),
                                        -1.0, 1.0));
     real u = relative_azimuth / k->pi;
@@ -929,7 +954,7 @@ EXPORT int oracle_build_multi_scattering(const AtmoParams* p, real bottom,
             real dc = 1.0 - 2.0 * sample_index / (real)direction_count;
             real ds = sqrt(fmax(1.0 - dc * dc, 0.0));
             real azimuth = (real)di * 2.39996323;
-            v3 ray = V(ds * cos(azimuth), dc, ds * sin(azimuth));
+            v3 ray = V(ds * cr_cos(azimuth), dc, ds * cr_sin(azimuth));
             real start, end;
             int valid, ground;
             atmosphere_segment(&k, origin, ray, bottom, &start, &end, &valid, &ground);
@@ -998,9 +1023,9 @@ EXPORT int oracle_build_sky_view(const AtmoParams* p, real bottom,
         real azimuth = ((real)x + 0.5) / (real)k.sw * k.pi;
         real view_v = ((real)y + 0.5) / (real)k.sh;
         real zenith = sky_v_to_zenith(&k, view_v, safe_alt, bottom);
-        real vc = cos(zenith);
-        real vsn = sin(zenith);
-        v3 ray = V(vsn * cos(azimuth), vc, vsn * sin(azimuth));
+        real vc = cr_cos(zenith);
+        real vsn = cr_sin(zenith);
+        v3 ray = V(vsn * cr_cos(azimuth), vc, vsn * cr_sin(azimuth));
         real start, end;
         int valid, ground;
         atmosphere_segment_from_altitude(&k, safe_alt, vc, bottom, &start, &end,
@@ -1185,9 +1210,9 @@ EXPORT int oracle_composite(
                                   0.0, 1.0);
                 vis = vis * vis * (3.0 - 2.0 * vis);
                 color = vadd(vadd(sky, vmul(color, trans)), vs(star, vis));
-                real angular_distance = acos(clampr(vdot(ray, sun), -1.0, 1.0));
+                real angular_distance = cr_acos(clampr(vdot(ray, sun), -1.0, 1.0));
                 real pixel_filter =
-                    atan2(tangent_half_fov * 1.41421356 / (real)height, (real)1.0);
+                    cr_atan2(tangent_half_fov * 1.41421356 / (real)height, (real)1.0);
                 real coverage = clampr((angular_radius + pixel_filter - angular_distance)
                                            / fmax(2.0 * pixel_filter, 1.0e-8),
                                        0.0, 1.0);

@@ -40,7 +40,37 @@ EXPORTED_SYMBOLS = (
     "atmo_build_aerial_perspective",
     "atmo_prepare_surface_lighting",
     "atmo_composite",
+    "atmo_mlp_create",
+    "atmo_mlp_destroy",
+    "atmo_mlp_last_error",
+    "atmo_mlp_set_params",
+    "atmo_mlp_get_params",
+    "atmo_mlp_set_transform",
+    "atmo_mlp_encode",
+    "atmo_mlp_forward",
+    "atmo_mlp_gradients",
+    "atmo_mlp_train_step",
+    "atmo_mlp_pack",
+    "atmo_mlp_infer",
+    "atmo_surrogate_labels",
+    "atmo_composite_surrogate",
 )
+
+MLP_PARAMS = 9094
+
+
+class AtmoSurrogateTransform(ctypes.Structure):
+    """struct AtmoSurrogateTransform (include/atmo.h)."""
+
+    _fields_ = [
+        ("altitude_max_m", ctypes.c_float),
+        ("distance_min_m", ctypes.c_float),
+        ("distance_max_m", ctypes.c_float),
+        ("out_mean", ctypes.c_float * 6),
+        ("out_scale", ctypes.c_float * 6),
+        ("eps_inscatter", ctypes.c_float),
+        ("eps_transmittance", ctypes.c_float),
+    ]
 
 
 class AtmoParams(ctypes.Structure):
@@ -105,8 +135,27 @@ def _signatures(lib: ctypes.CDLL) -> None:
         _P, _P, _P, _P, _P, _P, _F, _F, _F, _PF3, _PF3, _PF3, _F, _PF3, _PF3, _PF3,
         _F, _F, _F, _I, _P,
     ]
+    _PI = ctypes.POINTER(ctypes.c_int32)
+    lib.atmo_mlp_create.argtypes = [_I, ctypes.POINTER(_P)]
+    lib.atmo_mlp_destroy.argtypes = [_P]
+    lib.atmo_mlp_last_error.argtypes = [_P]
+    lib.atmo_mlp_last_error.restype = ctypes.c_char_p
+    lib.atmo_mlp_set_params.argtypes = [_P, _PF3, ctypes.c_int64]
+    lib.atmo_mlp_get_params.argtypes = [_P, _PF3, ctypes.c_int64]
+    lib.atmo_mlp_set_transform.argtypes = [_P, ctypes.POINTER(AtmoSurrogateTransform)]
+    lib.atmo_mlp_encode.argtypes = [_P, _P, ctypes.c_int64, _P, _P]
+    lib.atmo_mlp_forward.argtypes = [_P, _P, _I, _P, _P]
+    lib.atmo_mlp_gradients.argtypes = [_P, _P, _P, _P, _I, _P, _P, _P]
+    lib.atmo_mlp_train_step.argtypes = [_P, _P, _P, _P, _I, _F, _I, _P, _P]
+    lib.atmo_mlp_pack.argtypes = [_P, _P]
+    lib.atmo_mlp_infer.argtypes = [_P, _P, ctypes.c_int64, _P, _P]
+    lib.atmo_surrogate_labels.argtypes = [_P, _P, ctypes.c_int64, _F, _F, _P, _P]
+    lib.atmo_composite_surrogate.argtypes = [
+        _P, _P, _P, _P, _P, _P, _P, _F, _F, _F, _PF3, _PF3, _PF3, _F, _PF3, _PF3, _PF3,
+        _F, _F, _F, _P,
+    ]
     for name in EXPORTED_SYMBOLS:
-        if name != "atmo_last_error":
+        if name not in ("atmo_last_error", "atmo_mlp_last_error"):
             getattr(lib, name).restype = ctypes.c_int32
 
 

@@ -8,6 +8,14 @@
 // evaluation of the reference formulas is the ulp-level error of the
 // device exp/acos/cos/sin/pow/atan2.
 //
+// Elementary functions are correctly rounded float32 (cr_*: evaluated in
+// float64, rounded once), the same definition the CPU restatement uses.
+// The reference's formulas amplify last-bit differences of exp / acos / cos
+// by up to ~1e3 (finite-disk visibility near the horizon, (1 - exp(-x)) / s
+// for small x, the sky-view horizon coordinate), so a float32 libm whose
+// results differ in the last bit from another's cannot meet a 1e-4 parity
+// bar there; correctly rounded results are unique.
+//
 // Frame- and LUT-sized arrays use the reference's Taichi field layout:
 // (X, Y[, Z], 3) float32, element [x, y] at (x * Y + y) * 3.
 #pragma once
@@ -38,6 +46,13 @@ struct f3 {
     float x, y, z;
 };
 
+__device__ __forceinline__ float cr_exp(float x) { return (float)exp((double)x); }
+__device__ __forceinline__ float cr_acos(float x) { return (float)acos((double)x); }
+__device__ __forceinline__ float cr_cos(float x) { return (float)cos((double)x); }
+__device__ __forceinline__ float cr_sin(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ float cr_pow(float x, float y) { return (float)pow((double)x, (double)y); }
+__device__ __forceinline__ float cr_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
+
 __device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
 __device__ __forceinline__ f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
 __device__ __forceinline__ f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
@@ -59,7 +74,7 @@ __device__ __forceinline__ f3 mins(f3 a, float s) { return f3{fminf(a.x, s), fmi
 __device__ __forceinline__ f3 clamp3(f3 a, float lo, float hi) {
     return f3{clampf(a.x, lo, hi), clampf(a.y, lo, hi), clampf(a.z, lo, hi)};
 }
-__device__ __forceinline__ f3 exp3(f3 a) { return f3{expf(a.x), expf(a.y), expf(a.z)}; }
+__device__ __forceinline__ f3 exp3(f3 a) { return f3{cr_exp(a.x), cr_exp(a.y), cr_exp(a.z)}; }
 __device__ __forceinline__ f3 arr3(const float* a) { return f3{a[0], a[1], a[2]}; }
 __device__ __forceinline__ f3 ld3(const float* __restrict__ base, long i) {
     return f3{base[i * 3 + 0], base[i * 3 + 1], base[i * 3 + 2]};
@@ -76,8 +91,8 @@ __device__ __forceinline__ void st3(float* __restrict__ base, long i, f3 v) {
 __device__ __forceinline__ f3 density(const Consts& k, float h) {
     float rayleigh = 0.0f, mie = 0.0f, absorption = 0.0f;
     if (0.0f <= h && h <= k.top) {
-        rayleigh = expf(-h / k.h_r);
-        mie = expf(-h / k.h_m);
+        rayleigh = cr_exp(-h / k.h_r);
+        mie = cr_exp(-h / k.h_m);
         absorption = fmaxf(1.0f - fabsf(h - k.peak) / k.half_width, 0.0f);
     }
     return f3{rayleigh, mie, absorption};
@@ -304,7 +319,7 @@ __device__ __forceinline__ float sky_horizon_zenith(const Consts& k, float h, fl
     float a = fmaxf(h, 0.0f);
     float radius = bottom + a;
     float tc = shell_rho(a, bottom) / fmaxf(radius, 1.0f);
-    return k.pi - acosf(clampf(tc, 0.0f, 1.0f));
+    return k.pi - cr_acos(clampf(tc, 0.0f, 1.0f));
 }
 
 // renderer.py:492-504
@@ -431,11 +446,11 @@ __device__ __forceinline__ SunGeom sun_geometry(const Consts& k, float h, float 
     SunGeom g;
     float radius = bottom + fmaxf(h, 0.0f);
     float hc = -shell_rho(h, bottom) / fmaxf(radius, 1.0f);
-    g.horizon_zenith = acosf(clampf(hc, -1.0f, 1.0f));
-    g.sun_zenith = acosf(clampf(sun_cosine, -1.0f, 1.0f));
+    g.horizon_zenith = cr_acos(clampf(hc, -1.0f, 1.0f));
+    g.sun_zenith = cr_acos(clampf(sun_cosine, -1.0f, 1.0f));
     float sep = g.horizon_zenith - g.sun_zenith;
     float n = clampf(sep / fmaxf(angular_radius, 1.0e-6f), -1.0f, 1.0f);
-    g.visibility = (acosf(-n) + n * sqrtf(fmaxf(1.0f - n * n, 0.0f))) / k.pi;
+    g.visibility = (cr_acos(-n) + n * sqrtf(fmaxf(1.0f - n * n, 0.0f))) / k.pi;
     return g;
 }
 
@@ -444,7 +459,7 @@ __device__ __forceinline__ f3 solar_transmittance(const Consts& k, const float* 
                                                   const SunGeom& g, float h, float bottom,
                                                   float angular_radius) {
     float rep = fminf(g.sun_zenith, g.horizon_zenith - angular_radius * 0.25f);
-    float rc = cosf(fmaxf(rep, 0.0f));
+    float rc = cr_cos(fmaxf(rep, 0.0f));
     return sample_transmittance(k, tlut, h, rc, bottom) * g.visibility;
 }
 
@@ -456,7 +471,7 @@ __device__ __forceinline__ float rayleigh_phase(const Consts& k, float c) {
 // renderer.py:795-804
 __device__ __forceinline__ float mie_phase(const Consts& k, float c) {
     float den = fmaxf(k.mie_a - k.mie_b * c, 1.0e-8f);
-    return k.mie_c * (1.0f + c * c) / (k.mie_d * powf(den, 1.5f));
+    return k.mie_c * (1.0f + c * c) / (k.mie_d * cr_pow(den, 1.5f));
 }
 
 // renderer.py:806-816
@@ -665,12 +680,12 @@ __device__ __forceinline__ f3 camera_transmittance(const Consts& k,
 // renderer.py:1474-1525
 __device__ __forceinline__ f3 sample_sky_view(const Consts& k, const float* __restrict__ svlut,
                                               f3 ray, f3 sun, float h, float bottom) {
-    float zenith = acosf(clampf(ray.y, -1.0f, 1.0f));
+    float zenith = cr_acos(clampf(ray.y, -1.0f, 1.0f));
     float vl = sqrtf(ray.x * ray.x + ray.z * ray.z);
     float sl = sqrtf(sun.x * sun.x + sun.z * sun.z);
     float rel = 0.0f;
     if (vl > 1.0e-6f && sl > 1.0e-6f)
-        rel = acosf(clampf((ray.x * sun.x + ray.z * sun.z) / (vl * sl), -1.0f, 1.0f));
+        rel = cr_acos(clampf((ray.x * sun.x + ray.z * sun.z) / (vl * sl), -1.0f, 1.0f));
     float u = rel / k.pi;
     float v = sky_zenith_to_v(k, zenith, sky_horizon_zenith(k, h, bottom));
     return bilinear(svlut, k.sw, k.sh, u, v);

@@ -25,31 +25,9 @@
 
 #include "../../include/atmo.h"
 #include "atmo_device.h"
+#include "atmo_handle.h"
 
 using namespace atmo;
-
-namespace {
-
-constexpr uint32_t kMagic = 0x41544d4fu;  // "ATMO"
-
-}  // namespace
-
-struct AtmoHandle {
-    uint32_t magic;
-    AtmoParams params;
-    Consts k;
-    int width;
-    int height;
-    int device;
-    float* tlut;
-    float* mslut;
-    float* svlut;
-    float* aplut;
-    float* sun_t;
-    float* sky_l;
-    bool borrowed[6];  // buffer supplied by the caller through atmo_bind_buffer
-    char error[512];
-};
 
 namespace {
 
@@ -60,16 +38,6 @@ void set_error(AtmoHandle* h, const char* fmt, const char* a = "", const char* b
     snprintf(dst, 512, fmt, a, b);
 }
 
-bool handle_ok(const AtmoHandle* h) { return h != nullptr && h->magic == kMagic; }
-
-int32_t launch_status(AtmoHandle* h, const char* what) {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-        set_error(h, "%s: %s", what, hipGetErrorString(e));
-        return ATMO_EHIP;
-    }
-    return ATMO_OK;
-}
 
 // AtmosphereConfig.__post_init__ (config.py:61-128), same messages.
 int32_t validate(const AtmoParams* p, char* msg, size_t cap) {
@@ -271,7 +239,7 @@ __global__ void __launch_bounds__(256) k_multi_scattering(Consts k, float bottom
         float dc = 1.0f - 2.0f * si / (float)N;
         float ds = sqrtf(fmaxf(1.0f - dc * dc, 0.0f));
         float az = (float)di * 2.39996323f;
-        f3 ray = mk(ds * cosf(az), dc, ds * sinf(az));
+        f3 ray = mk(ds * cr_cos(az), dc, ds * cr_sin(az));
         float start, end;
         bool valid, ground;
         segment_vec(k, origin, ray, bottom, start, end, valid, ground);
@@ -347,9 +315,9 @@ __global__ void __launch_bounds__(256) k_sky_view(Consts k, float bottom, float 
     float az = ((float)x + 0.5f) / (float)k.sw * k.pi;
     float view_v = ((float)y + 0.5f) / (float)k.sh;
     float zenith = sky_v_to_zenith(k, view_v, sky_horizon_zenith(k, safe_alt, bottom));
-    float vc = cosf(zenith);
-    float vsn = sinf(zenith);
-    f3 ray = mk(vsn * cosf(az), vc, vsn * sinf(az));
+    float vc = cr_cos(zenith);
+    float vsn = cr_sin(zenith);
+    f3 ray = mk(vsn * cr_cos(az), vc, vsn * cr_sin(az));
     float start, end;
     bool valid, ground;
     segment_from_altitude(k, safe_alt, vc, bottom, start, end, valid, ground);
@@ -533,8 +501,8 @@ __global__ void __launch_bounds__(256) k_composite(Consts k, CompositeArgs a) {
         float vis = clampf((contrast - a.c0) / fmaxf(a.c1 - a.c0, 1.0e-5f), 0.0f, 1.0f);
         vis = vis * vis * (3.0f - 2.0f * vis);
         color = (sky + color * trans) + star * vis;
-        float angular_distance = acosf(clampf(dot(ray, a.sun), -1.0f, 1.0f));
-        float pixel_filter = atan2f(a.thf * 1.41421356f / (float)a.height, 1.0f);
+        float angular_distance = cr_acos(clampf(dot(ray, a.sun), -1.0f, 1.0f));
+        float pixel_filter = cr_atan2(a.thf * 1.41421356f / (float)a.height, 1.0f);
         float cov = clampf((a.ar + pixel_filter - angular_distance) / fmaxf(2.0f * pixel_filter, 1.0e-8f),
                            0.0f, 1.0f);
         cov = cov * cov * (3.0f - 2.0f * cov);
@@ -679,7 +647,7 @@ int32_t atmo_create(const AtmoParams* params, int32_t width, int32_t height, Atm
         set_error(nullptr, "atmo_create: host allocation failed");
         return ATMO_ENOMEM;
     }
-    h->magic = kMagic;
+    h->magic = kAtmoMagic;
     h->params = *params;
     h->k = make_consts(*params);
     h->width = width;
@@ -715,7 +683,7 @@ int32_t atmo_create(const AtmoParams* params, int32_t width, int32_t height, Atm
 }
 
 int32_t atmo_destroy(AtmoHandle* h) {
-    if (!handle_ok(h)) return ATMO_EHANDLE;
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
     free_buffers(h);
     h->magic = 0;
     delete h;
@@ -724,12 +692,12 @@ int32_t atmo_destroy(AtmoHandle* h) {
 
 const char* atmo_last_error(const AtmoHandle* h) {
     if (!h) return g_create_error;
-    if (!handle_ok(h)) return "invalid atmosphere handle";
+    if (!atmo_handle_ok(h)) return "invalid atmosphere handle";
     return h->error;
 }
 
 int32_t atmo_get_buffer(AtmoHandle* h, int32_t which, void** dptr, int64_t shape[4]) {
-    if (!handle_ok(h)) return ATMO_EHANDLE;
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
     if (!dptr || !shape) {
         set_error(h, "atmo_get_buffer: NULL argument");
         return ATMO_EINVAL;
@@ -744,7 +712,7 @@ int32_t atmo_get_buffer(AtmoHandle* h, int32_t which, void** dptr, int64_t shape
 }
 
 int32_t atmo_bind_buffer(AtmoHandle* h, int32_t which, void* dptr, int64_t bytes) {
-    if (!handle_ok(h)) return ATMO_EHANDLE;
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
     if (which < 0 || which > 5) {
         set_error(h, "atmo_bind_buffer: unknown buffer id");
         return ATMO_EINVAL;
@@ -770,16 +738,16 @@ int32_t atmo_bind_buffer(AtmoHandle* h, int32_t which, void* dptr, int64_t bytes
 }
 
 int32_t atmo_build_transmittance(AtmoHandle* h, float bottom, void* stream) {
-    if (!handle_ok(h)) return ATMO_EHANDLE;
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
     const long threads = (long)h->k.tw * h->k.th * kGroupT;
     hipLaunchKernelGGL(k_transmittance<kGroupT>, dim3(blocks_for(threads, 256)), dim3(256), 0,
                        (hipStream_t)stream, h->k, bottom, h->tlut);
-    return launch_status(h, "atmo_build_transmittance");
+    return atmo_launch_status(h, "atmo_build_transmittance");
 }
 
 int32_t atmo_build_multi_scattering(AtmoHandle* h, float bottom, const float irradiance[3],
                                     float ar, void* stream) {
-    if (!handle_ok(h)) return ATMO_EHANDLE;
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
     if (!irradiance) {
         set_error(h, "atmo_build_multi_scattering: NULL irradiance");
         return ATMO_EINVAL;
@@ -788,12 +756,12 @@ int32_t atmo_build_multi_scattering(AtmoHandle* h, float bottom, const float irr
     hipLaunchKernelGGL(k_multi_scattering, dim3(blocks_for(threads, 256)), dim3(256), 0,
                        (hipStream_t)stream, h->k, bottom, host3(irradiance), ar, h->tlut,
                        h->mslut);
-    return launch_status(h, "atmo_build_multi_scattering");
+    return atmo_launch_status(h, "atmo_build_multi_scattering");
 }
 
 int32_t atmo_build_sky_view(AtmoHandle* h, float bottom, float cam_alt, float sun_zc,
                             const float irradiance[3], float ar, void* stream) {
-    if (!handle_ok(h)) return ATMO_EHANDLE;
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
     if (!irradiance) {
         set_error(h, "atmo_build_sky_view: NULL irradiance");
         return ATMO_EINVAL;
@@ -802,14 +770,14 @@ int32_t atmo_build_sky_view(AtmoHandle* h, float bottom, float cam_alt, float su
     hipLaunchKernelGGL(k_sky_view<kGroupSV>, dim3(blocks_for(threads, 256)), dim3(256), 0,
                        (hipStream_t)stream, h->k, bottom, cam_alt, sun_zc, host3(irradiance), ar,
                        h->tlut, h->mslut, h->svlut);
-    return launch_status(h, "atmo_build_sky_view");
+    return atmo_launch_status(h, "atmo_build_sky_view");
 }
 
 int32_t atmo_build_aerial_perspective(AtmoHandle* h, float bottom, float cam_alt,
                                       const float sun[3], const float irradiance[3], float ar,
                                       const float right[3], const float up[3],
                                       const float forward[3], float thf, void* stream) {
-    if (!handle_ok(h)) return ATMO_EHANDLE;
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
     if (!sun || !irradiance || !right || !up || !forward) {
         set_error(h, "atmo_build_aerial_perspective: NULL vector argument");
         return ATMO_EINVAL;
@@ -819,14 +787,14 @@ int32_t atmo_build_aerial_perspective(AtmoHandle* h, float bottom, float cam_alt
                        (hipStream_t)stream, h->k, h->width, h->height, bottom, cam_alt,
                        host3(sun), host3(irradiance), ar, host3(right), host3(up),
                        host3(forward), thf, h->tlut, h->mslut, h->aplut);
-    return launch_status(h, "atmo_build_aerial_perspective");
+    return atmo_launch_status(h, "atmo_build_aerial_perspective");
 }
 
 int32_t atmo_prepare_surface_lighting(AtmoHandle* h, const float* position_view,
                                       const int32_t* surface_id, float bottom, float cam_alt,
                                       const float sun[3], float ar, const float right[3],
                                       const float up[3], const float forward[3], void* stream) {
-    if (!handle_ok(h)) return ATMO_EHANDLE;
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
     if (!position_view || !surface_id || !sun || !right || !up || !forward) {
         set_error(h, "atmo_prepare_surface_lighting: NULL argument");
         return ATMO_EINVAL;
@@ -836,7 +804,7 @@ int32_t atmo_prepare_surface_lighting(AtmoHandle* h, const float* position_view,
                        h->k, npix, position_view, surface_id, bottom, cam_alt, host3(sun), ar,
                        host3(right), host3(up), host3(forward), h->tlut, h->mslut, h->sun_t,
                        h->sky_l);
-    return launch_status(h, "atmo_prepare_surface_lighting");
+    return atmo_launch_status(h, "atmo_prepare_surface_lighting");
 }
 
 int32_t atmo_composite(AtmoHandle* h, const float* scene_hdr, const float* space_hdr,
@@ -845,7 +813,7 @@ int32_t atmo_composite(AtmoHandle* h, const float* scene_hdr, const float* space
                        const float irradiance[3], const float disk[3], float ar,
                        const float right[3], const float up[3], const float forward[3], float thf,
                        float c0, float c1, int32_t view, void* stream) {
-    if (!handle_ok(h)) return ATMO_EHANDLE;
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
     if (!scene_hdr || !space_hdr || !surface_id || !position_view || !hdr || !sun ||
         !irradiance || !disk || !right || !up || !forward) {
         set_error(h, "atmo_composite: NULL argument");
@@ -888,7 +856,7 @@ int32_t atmo_composite(AtmoHandle* h, const float* scene_hdr, const float* space
     else
         hipLaunchKernelGGL(k_composite_diagnostic, dim3(blocks_for(npix, 256)), dim3(256), 0,
                            (hipStream_t)stream, h->k, a);
-    return launch_status(h, "atmo_composite");
+    return atmo_launch_status(h, "atmo_composite");
 }
 
 }  // extern "C"

@@ -1,0 +1,807 @@
+// surrogate.hip -- the learned aerial-perspective surrogate of libatmo.so.
+//
+// A 5 -> 64 -> 64 -> 64 -> 6 SiLU MLP maps (camera altitude, view-zenith
+// cosine, sun-zenith cosine, relative azimuth, segment length) to the RGB
+// in-scatter per unit irradiance and the RGB transmittance of the
+// composite's surface branch (renderer.py:1603-1786, 1863-1880).  This file
+// holds:
+//  - label generation by the GPU integrator itself (the surface_inscatter /
+//    surface_transmittance device functions of the plain composite);
+//  - FP32 training kernels: dense forward, backward (data and weights, the
+//    weight gradient reduced over the batch in two deterministic stages)
+//    and Adam;
+//  - the surrogate composite: BF16 MFMA inference (v_mfma_f32_16x16x16bf16,
+//    FP32 accumulation) for in-domain surface pixels, the plain integrator
+//    for out-of-domain surface pixels, the plain sky branch for sky pixels.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <new>
+
+#include "atmo_device.h"
+#include "atmo_handle.h"
+
+using namespace atmo;
+
+namespace {
+
+constexpr uint32_t kMlpMagic = 0x4d4c5053u;  // "MLPS"
+constexpr int kChunk = 128;                  // samples per weight-gradient partial
+
+struct Layer {
+    int w, b, in, out;
+};
+constexpr Layer kLayers[4] = {
+    {0, 320, 5, 64},
+    {384, 4480, 64, 64},
+    {4544, 8640, 64, 64},
+    {8704, 9088, 64, 6},
+};
+
+}  // namespace
+
+struct AtmoMlp {
+    uint32_t magic;
+    int max_batch;
+    float* params;  // fp32 master weights [ATMO_MLP_PARAMS]
+    float* grads;
+    float* adam_m;
+    float* adam_v;
+    float* z[4];     // pre-activations per layer [B, out]
+    float* a[3];     // hidden activations silu(z) [B, 64]
+    float* dz[4];    // d loss / d z per layer [B, out]
+    float* xbuf;     // gathered inputs [B, 5]
+    float* partial;  // weight-gradient partial sums [B / kChunk, 64 * 64 + 64]
+    float* loss;     // [B / 256 + 1] block partials, loss at [0] after reduce
+    uint16_t* frag;  // bf16 MFMA weight fragments for inference
+    float* bias;     // fp32 biases in inference order [64 + 64 + 64 + 16]
+    AtmoSurrogateTransform transform;
+    char error[512];
+};
+
+namespace {
+
+bool mlp_ok(const AtmoMlp* m) { return m != nullptr && m->magic == kMlpMagic; }
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+
+__device__ __forceinline__ float dsilu(float z) {
+    float s = 1.0f / (1.0f + expf(-z));
+    return s * (1.0f + z * (1.0f - s));
+}
+
+// ------------------------------------------------------------- features
+
+// Canonical geometry of one sample: the in-scatter and transmittance of a
+// ray from altitude h depend on the ray and the sun only through their
+// zenith cosines and relative azimuth (spherically symmetric shell, origin
+// on the local vertical).
+__device__ __forceinline__ void sample_geometry(float mu, float mu_s, float phi, f3& ray, f3& sun) {
+    float sr = sqrtf(fmaxf(1.0f - mu * mu, 0.0f));
+    float ss = sqrtf(fmaxf(1.0f - mu_s * mu_s, 0.0f));
+    ray = mk(sr, mu, 0.0f);
+    sun = mk(ss * cr_cos(phi), mu_s, ss * cr_sin(phi));
+}
+
+__device__ __forceinline__ void encode(const AtmoSurrogateTransform& t, float h, float mu,
+                                       float mu_s, float phi, float d, float x[5]) {
+    x[0] = h * (2.0f / t.altitude_max_m) - 1.0f;
+    x[1] = mu;
+    x[2] = mu_s;
+    x[3] = phi * (2.0f / 3.14159265358979f) - 1.0f;
+    float ld0 = logf(t.distance_min_m), ld1 = logf(t.distance_max_m);
+    x[4] = (logf(d) - ld0) * (2.0f / (ld1 - ld0)) - 1.0f;
+}
+
+__device__ __forceinline__ bool in_domain(const AtmoSurrogateTransform& t, float h, float d) {
+    return h >= 0.0f && h <= t.altitude_max_m && d >= t.distance_min_m && d <= t.distance_max_m;
+}
+
+// ---------------------------------------------------------------- labels
+
+__global__ void __launch_bounds__(256) k_labels(Consts k, long n, const float* __restrict__ raw,
+                                                float bottom, float ar,
+                                                const float* __restrict__ tlut,
+                                                const float* __restrict__ mslut,
+                                                float* __restrict__ labels) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* s = raw + i * 5;
+    f3 ray, sun;
+    sample_geometry(s[1], s[2], s[3], ray, sun);
+    const f3 unit = mk(1.0f, 1.0f, 1.0f);
+    f3 L = surface_inscatter(k, tlut, mslut, s[0], ray, s[4], sun, unit, bottom, ar);
+    f3 T = surface_transmittance(k, tlut, s[0], ray, s[4], bottom);
+    float* o = labels + i * 6;
+    o[0] = L.x;
+    o[1] = L.y;
+    o[2] = L.z;
+    o[3] = T.x;
+    o[4] = T.y;
+    o[5] = T.z;
+}
+
+__global__ void __launch_bounds__(256) k_encode(long n, const float* __restrict__ raw,
+                                                AtmoSurrogateTransform t, float* __restrict__ x) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* s = raw + i * 5;
+    float f[5];
+    encode(t, s[0], s[1], s[2], s[3], s[4], f);
+    for (int q = 0; q < 5; ++q) x[i * 5 + q] = f[q];
+}
+
+// ------------------------------------------------------------- training
+
+// Gathers rows idx[s] of src [*, width] into dst [B, width] (idx may be NULL).
+__global__ void __launch_bounds__(256) k_gather(int B, int width, const int32_t* __restrict__ idx,
+                                                const float* __restrict__ src,
+                                                float* __restrict__ dst) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)B * width) return;
+    const int s = (int)(e / width), c = (int)(e % width);
+    const long row = idx ? (long)idx[s] : (long)s;
+    dst[e] = src[row * width + c];
+}
+
+// z[s, n] = b[n] + sum_k x[s, k] W[n, k]; a = silu(z) when `a` is non-null.
+// Thread per (sample, neuron); W staged transposed in LDS.
+__global__ void __launch_bounds__(256) k_dense_forward(int B, int K, int N, const float* __restrict__ x,
+                                                       const float* __restrict__ W,
+                                                       const float* __restrict__ bias,
+                                                       float* __restrict__ z, float* __restrict__ a) {
+    __shared__ float wt[64 * 64];
+    for (int e = threadIdx.x; e < K * N; e += blockDim.x) {
+        int n = e / K, kk = e % K;
+        wt[kk * N + n] = W[e];
+    }
+    __syncthreads();
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)B * N) return;
+    const int s = (int)(e / N), n = (int)(e % N);
+    const float* xs = x + (long)s * K;
+    float acc = bias[n];
+    for (int kk = 0; kk < K; ++kk) acc += xs[kk] * wt[kk * N + n];
+    z[e] = acc;
+    if (a) a[e] = silu(acc);
+}
+
+// Mean-square-error gradient dz = 2 (y - t) / (B * 6) and per-block loss sums.
+// `t` and `dz` may alias (the target is overwritten by its gradient).
+__global__ void __launch_bounds__(256) k_mse(int B, const float* __restrict__ y, const float* t,
+                                             float* dz, float* __restrict__ block_loss) {
+    __shared__ float red[256];
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)B * 6;
+    float d = 0.0f;
+    if (e < n) {
+        d = y[e] - t[e];
+        dz[e] = 2.0f * d / (float)n;
+    }
+    red[threadIdx.x] = d * d;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) block_loss[blockIdx.x + 1] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_loss_reduce(int blocks, long count, float* __restrict__ block_loss) {
+    __shared__ float red[256];
+    float acc = 0.0f;
+    for (int b = threadIdx.x; b < blocks; b += blockDim.x) acc += block_loss[b + 1];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) block_loss[0] = red[0] / (float)count;
+}
+
+// dz_prev[s, k] = (sum_n dz[s, n] W[n, k]) * silu'(z_prev[s, k]).
+__global__ void __launch_bounds__(256) k_dense_backward_data(int B, int N, int K,
+                                                             const float* __restrict__ dz,
+                                                             const float* __restrict__ W,
+                                                             const float* __restrict__ z_prev,
+                                                             float* __restrict__ dz_prev) {
+    __shared__ float w[64 * 64];
+    for (int e = threadIdx.x; e < K * N; e += blockDim.x) w[e] = W[e];
+    __syncthreads();
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)B * K) return;
+    const int s = (int)(e / K), kk = (int)(e % K);
+    const float* d = dz + (long)s * N;
+    float acc = 0.0f;
+    for (int n = 0; n < N; ++n) acc += d[n] * w[n * K + kk];
+    dz_prev[e] = acc * dsilu(z_prev[e]);
+}
+
+// Partial weight / bias gradients of one chunk of kChunk samples:
+// partial[c, n * K + k] = sum_s dz[s, n] x[s, k], partial[c, N * K + n] = sum_s dz[s, n].
+__global__ void __launch_bounds__(256) k_dense_backward_weights(int B, int N, int K,
+                                                                const float* __restrict__ dz,
+                                                                const float* __restrict__ x,
+                                                                float* __restrict__ partial) {
+    __shared__ float dzs[kChunk * 64];
+    __shared__ float xs[kChunk * 64];
+    const int c = blockIdx.x;
+    const int s0 = c * kChunk;
+    const int rows = min(kChunk, B - s0);
+    for (int e = threadIdx.x; e < rows * N; e += blockDim.x) dzs[e] = dz[(long)s0 * N + e];
+    for (int e = threadIdx.x; e < rows * K; e += blockDim.x) xs[e] = x[(long)s0 * K + e];
+    __syncthreads();
+    const int count = N * K + N;
+    float* out = partial + (long)c * count;
+    for (int e = threadIdx.x; e < count; e += blockDim.x) {
+        float acc = 0.0f;
+        if (e < N * K) {
+            const int n = e / K, kk = e % K;
+            for (int s = 0; s < rows; ++s) acc += dzs[s * N + n] * xs[s * K + kk];
+        } else {
+            const int n = e - N * K;
+            for (int s = 0; s < rows; ++s) acc += dzs[s * N + n];
+        }
+        out[e] = acc;
+    }
+}
+
+// grad[e] = sum over chunks of partial[c, e], in chunk order.
+__global__ void __launch_bounds__(256) k_reduce_partials(int chunks, int count,
+                                                         const float* __restrict__ partial,
+                                                         float* __restrict__ grad) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    float acc = 0.0f;
+    for (int c = 0; c < chunks; ++c) acc += partial[(long)c * count + e];
+    grad[e] = acc;
+}
+
+// Adam (Kingma & Ba 2015, Algorithm 1) with bias corrections bc1 = 1 - b1^t,
+// bc2 = 1 - b2^t.
+__global__ void __launch_bounds__(256) k_adam(int n, float* __restrict__ p, const float* __restrict__ g,
+                                              float* __restrict__ m, float* __restrict__ v, float lr,
+                                              float b1, float b2, float eps, float bc1, float bc2) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    float gi = g[e];
+    float mi = b1 * m[e] + (1.0f - b1) * gi;
+    float vi = b2 * v[e] + (1.0f - b2) * gi * gi;
+    m[e] = mi;
+    v[e] = vi;
+    p[e] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+// -------------------------------------------------------- MFMA inference
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return (short)(u >> 16);
+}
+
+__host__ __device__ constexpr int frag_index(int layer, int mt, int kt) {
+    // layer 0: 4 x 1, layers 1-2: 4 x 4, layer 3: 1 x 4 fragments
+    return layer == 0 ? mt : layer == 1 ? 4 + mt * 4 + kt : layer == 2 ? 20 + mt * 4 + kt : 36 + kt;
+}
+constexpr int kFragments = 40;
+
+// Packs fp32 parameters into per-lane bf16 A-operand fragments of
+// v_mfma_f32_16x16x16bf16_1k: for tile (mt, kt) lane l holds
+// W[16 mt + l % 16][16 kt + 4 (l / 16) + i], i = 0..3 (zero outside W).
+// Also gathers the biases in output-row order.
+__global__ void k_pack_fragments(const float* __restrict__ params, uint16_t* __restrict__ frag,
+                                 float* __restrict__ bias) {
+    const int lane = threadIdx.x;  // 64 threads
+    for (int layer = 0; layer < 4; ++layer) {
+        const Layer L = kLayers[layer];
+        const int mts = (L.out + 15) / 16, kts = (L.in + 15) / 16;
+        for (int mt = 0; mt < mts; ++mt)
+            for (int kt = 0; kt < kts; ++kt) {
+                uint16_t* f = frag + ((long)frag_index(layer, mt, kt) * 64 + lane) * 4;
+                const int row = 16 * mt + lane % 16;
+                for (int i = 0; i < 4; ++i) {
+                    const int col = 16 * kt + 4 * (lane / 16) + i;
+                    float w = (row < L.out && col < L.in) ? params[L.w + row * L.in + col] : 0.0f;
+                    f[i] = (uint16_t)to_bf16(w);
+                }
+            }
+    }
+    for (int e = lane; e < 64 * 3 + 16; e += 64) {
+        const int layer = e / 64 < 3 ? e / 64 : 3;
+        const int row = e - 64 * layer;
+        const Layer L = kLayers[layer];
+        bias[e] = row < L.out ? params[L.b + row] : 0.0f;
+    }
+}
+
+__device__ __forceinline__ floatx4 mfma(bf16x4 a, bf16x4 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x4 silu_bf16(floatx4 c) {
+    bf16x4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float z = c[i];
+        float e = __builtin_amdgcn_exp2f(-1.44269504f * z);
+        r[i] = to_bf16(z * __builtin_amdgcn_rcpf(1.0f + e));
+    }
+    return r;
+}
+
+// Runs the MLP for the 64 pixels of a wave.  `x` holds the lane's own
+// pixel features; returns the 6 raw outputs of the lane's pixel.  The
+// weights are the A operand and activations the B operand, so the output
+// tile of one layer (lane l: neurons 4 (l / 16) + i of pixel l % 16) is
+// exactly the B fragment of the next layer's K-step: no data movement.
+__device__ __forceinline__ void mlp_wave(const float x[5], const bf16x4* __restrict__ frag,
+                                         const float* __restrict__ bias, float out[6]) {
+    const int lane = threadIdx.x % 64;
+    const int group = lane / 16, col = lane % 16;
+#pragma unroll 1
+    for (int j = 0; j < 4; ++j) {
+        const int src = 16 * j + col;
+        float f[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) f[q] = __shfl(x[q], src, 64);
+        bf16x4 in;
+        in[0] = group == 0 ? to_bf16(f[0]) : group == 1 ? to_bf16(f[4]) : (short)0;
+        in[1] = group == 0 ? to_bf16(f[1]) : (short)0;
+        in[2] = group == 0 ? to_bf16(f[2]) : (short)0;
+        in[3] = group == 0 ? to_bf16(f[3]) : (short)0;
+        bf16x4 act[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            floatx4 c;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) c[i] = bias[16 * mt + 4 * group + i];
+            c = mfma(frag[frag_index(0, mt, 0) * 64 + lane], in, c);
+            act[mt] = silu_bf16(c);
+        }
+#pragma unroll
+        for (int layer = 1; layer <= 2; ++layer) {
+            bf16x4 next[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                floatx4 c;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) c[i] = bias[64 * layer + 16 * mt + 4 * group + i];
+#pragma unroll
+                for (int kt = 0; kt < 4; ++kt)
+                    c = mfma(frag[frag_index(layer, mt, kt) * 64 + lane], act[kt], c);
+                next[mt] = silu_bf16(c);
+            }
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) act[mt] = next[mt];
+        }
+        floatx4 c;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c[i] = bias[192 + 4 * group + i];
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) c = mfma(frag[frag_index(3, 0, kt) * 64 + lane], act[kt], c);
+        // outputs 0-3 of pixel 16 j + col sit in lane col, 4-5 in lane col + 16
+        float o[6];
+        o[0] = __shfl(c[0], col, 64);
+        o[1] = __shfl(c[1], col, 64);
+        o[2] = __shfl(c[2], col, 64);
+        o[3] = __shfl(c[3], col, 64);
+        o[4] = __shfl(c[0], col + 16, 64);
+        o[5] = __shfl(c[1], col + 16, 64);
+        if (group == j)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) out[q] = o[q];
+    }
+}
+
+__device__ __forceinline__ void decode(const AtmoSurrogateTransform& t, const float raw[6], f3& L,
+                                       f3& T) {
+    float y[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) y[q] = raw[q] * t.out_scale[q] + t.out_mean[q];
+    L = mk(fmaxf(expf(y[0]) - t.eps_inscatter, 0.0f), fmaxf(expf(y[1]) - t.eps_inscatter, 0.0f),
+           fmaxf(expf(y[2]) - t.eps_inscatter, 0.0f));
+    T = clamp3(mk(expf(y[3]) - t.eps_transmittance, expf(y[4]) - t.eps_transmittance,
+                  expf(y[5]) - t.eps_transmittance),
+               0.0f, 1.0f);
+}
+
+struct SurrogateArgs {
+    const float* scene;
+    const float* space;
+    const int32_t* sid;
+    const float* pos;
+    float* hdr;
+    const float* tlut;
+    const float* mslut;
+    const float* svlut;
+    int width, height;
+    float bottom, cam_alt, sky_alt;
+    f3 sun, irradiance, disk;
+    float ar;
+    f3 right, up, forward;
+    float thf, c0, c1;
+    const uint16_t* frag;
+    const float* bias;
+    AtmoSurrogateTransform t;
+};
+
+// The COMPOSITE view with the surrogate in place of the surface branch's
+// in-scatter and transmittance.  One thread per pixel, waves of 64
+// consecutive pixels (the block is 256 threads; every lane reaches the
+// MFMA calls, which need the whole wave).
+__global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, SurrogateArgs a) {
+    __shared__ bf16x4 frag_s[kFragments * 64];
+    __shared__ float bias_s[64 * 3 + 16];
+    const bf16x4* fg = reinterpret_cast<const bf16x4*>(a.frag);
+    for (int e = threadIdx.x; e < kFragments * 64; e += blockDim.x) frag_s[e] = fg[e];
+    for (int e = threadIdx.x; e < 64 * 3 + 16; e += blockDim.x) bias_s[e] = a.bias[e];
+    __syncthreads();
+
+    const long npix = (long)a.width * a.height;
+    const long i_raw = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i_raw < npix;
+    const long i = live ? i_raw : npix - 1;
+    const int px = (int)(i / a.height), py = (int)(i % a.height);
+    const float aspect = (float)a.width / (float)a.height;
+    const float su = ((float)px + 0.5f) / (float)a.width;
+    const float sv = ((float)py + 0.5f) / (float)a.height;
+    const float sx = (su * 2.0f - 1.0f) * aspect;
+    const float sy = sv * 2.0f - 1.0f;
+    const f3 ray = normalized((a.right * sx + a.up * sy) + a.forward / a.thf);
+    const bool surface = a.sid[i] >= 0;
+    const float sd = surface ? norm(ld3(a.pos, i)) : 1.0f;
+    const bool learned = live && surface && in_domain(a.t, a.cam_alt, sd);
+
+    // features of this lane's pixel (relative azimuth as in the sky-view
+    // sampler, renderer.py:1482-1497)
+    float x[5];
+    {
+        float vl = sqrtf(ray.x * ray.x + ray.z * ray.z);
+        float sl = sqrtf(a.sun.x * a.sun.x + a.sun.z * a.sun.z);
+        float phi = 0.0f;
+        if (vl > 1.0e-6f && sl > 1.0e-6f)
+            phi = acosf(clampf((ray.x * a.sun.x + ray.z * a.sun.z) / (vl * sl), -1.0f, 1.0f));
+        encode(a.t, a.cam_alt, clampf(ray.y, -1.0f, 1.0f), clampf(a.sun.y, -1.0f, 1.0f), phi,
+               fmaxf(sd, a.t.distance_min_m), x);
+    }
+    float raw[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    // the wave runs the network if any of its pixels needs it
+    if (__ballot(learned)) mlp_wave(x, frag_s, bias_s, raw);
+    if (!live) return;
+
+    f3 color = ld3(a.scene, i);
+    if (surface) {
+        f3 L, T;
+        if (learned) {
+            decode(a.t, raw, L, T);
+            L = L * a.irradiance;
+        } else {  // outside the training domain: the plain integrator
+            L = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun, a.irradiance,
+                                  a.bottom, a.ar);
+            T = surface_transmittance(k, a.tlut, a.cam_alt, ray, sd, a.bottom);
+        }
+        color = L + color * T;
+    } else {
+        f3 trans = camera_transmittance(k, a.tlut, a.cam_alt, ray, a.bottom);
+        f3 sky = sample_sky_view(k, a.svlut, ray, a.sun, a.sky_alt, a.bottom);
+        f3 star = ld3(a.space, i) * trans;
+        const f3 lum = mk(0.2126f, 0.7152f, 0.0722f);
+        float contrast = dot(star, lum) / fmaxf(dot(sky, lum), 1.0e-5f);
+        float vis = clampf((contrast - a.c0) / fmaxf(a.c1 - a.c0, 1.0e-5f), 0.0f, 1.0f);
+        vis = vis * vis * (3.0f - 2.0f * vis);
+        color = (sky + color * trans) + star * vis;
+        float angular_distance = cr_acos(clampf(dot(ray, a.sun), -1.0f, 1.0f));
+        float pixel_filter = cr_atan2(a.thf * 1.41421356f / (float)a.height, 1.0f);
+        float cov = clampf((a.ar + pixel_filter - angular_distance) / fmaxf(2.0f * pixel_filter, 1.0e-8f),
+                           0.0f, 1.0f);
+        cov = cov * cov * (3.0f - 2.0f * cov);
+        color = color + (a.disk * trans) * cov;
+    }
+    st3(a.hdr, i, maxs(color, 0.0f));
+}
+
+// Batch inference on encoded features (testing and evaluation): x [n, 5]
+// -> y [n, 6] raw network outputs, through the same MFMA path.
+__global__ void __launch_bounds__(256) k_mlp_infer(long n, const float* __restrict__ x,
+                                                   const uint16_t* __restrict__ frag,
+                                                   const float* __restrict__ bias,
+                                                   float* __restrict__ y) {
+    __shared__ bf16x4 frag_s[kFragments * 64];
+    __shared__ float bias_s[64 * 3 + 16];
+    const bf16x4* fg = reinterpret_cast<const bf16x4*>(frag);
+    for (int e = threadIdx.x; e < kFragments * 64; e += blockDim.x) frag_s[e] = fg[e];
+    for (int e = threadIdx.x; e < 64 * 3 + 16; e += blockDim.x) bias_s[e] = bias[e];
+    __syncthreads();
+    const long i_raw = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long i = i_raw < n ? i_raw : n - 1;
+    float f[5];
+    for (int q = 0; q < 5; ++q) f[q] = x[i * 5 + q];
+    float out[6];
+    mlp_wave(f, frag_s, bias_s, out);
+    if (i_raw < n)
+        for (int q = 0; q < 6; ++q) y[i * 6 + q] = out[q];
+}
+
+unsigned blocks_for(long threads, int block) { return (unsigned)((threads + block - 1) / block); }
+
+template <typename T>
+hipError_t alloc(T** p, size_t count) {
+    *p = nullptr;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+void free_mlp(AtmoMlp* m) {
+    void* bufs[] = {m->params, m->grads, m->adam_m, m->adam_v, m->z[0], m->z[1], m->z[2], m->z[3],
+                    m->a[0], m->a[1], m->a[2], m->dz[0], m->dz[1], m->dz[2], m->dz[3], m->xbuf,
+                    m->partial, m->loss, m->frag, m->bias};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+}
+
+// Forward pass of the training network on m->xbuf (B rows).
+void forward_layers(AtmoMlp* m, int B, hipStream_t s) {
+    const float* in = m->xbuf;
+    for (int l = 0; l < 4; ++l) {
+        const Layer L = kLayers[l];
+        hipLaunchKernelGGL(k_dense_forward, dim3(blocks_for((long)B * L.out, 256)), dim3(256), 0, s, B,
+                           L.in, L.out, in, m->params + L.w, m->params + L.b, m->z[l],
+                           l < 3 ? m->a[l] : nullptr);
+        if (l < 3) in = m->a[l];
+    }
+}
+
+// Backward pass from dz[3] (already holding d loss / d output).
+void backward_layers(AtmoMlp* m, int B, hipStream_t s) {
+    for (int l = 3; l >= 0; --l) {
+        const Layer L = kLayers[l];
+        const float* x = l == 0 ? m->xbuf : m->a[l - 1];
+        const int chunks = (B + kChunk - 1) / kChunk;
+        hipLaunchKernelGGL(k_dense_backward_weights, dim3(chunks), dim3(256), 0, s, B, L.out, L.in,
+                           m->dz[l], x, m->partial);
+        hipLaunchKernelGGL(k_reduce_partials, dim3(blocks_for(L.out * L.in + L.out, 256)), dim3(256),
+                           0, s, chunks, L.out * L.in + L.out, m->partial, m->grads + L.w);
+        if (l > 0)
+            hipLaunchKernelGGL(k_dense_backward_data, dim3(blocks_for((long)B * L.in, 256)), dim3(256),
+                               0, s, B, L.out, L.in, m->dz[l], m->params + L.w, m->z[l - 1],
+                               m->dz[l - 1]);
+    }
+}
+
+int32_t status(AtmoMlp* m, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        snprintf(m->error, sizeof m->error, "%s: %s", what, hipGetErrorString(e));
+        return ATMO_EHIP;
+    }
+    return ATMO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t atmo_mlp_create(int32_t max_batch, AtmoMlp** out) {
+    if (!out || max_batch <= 0 || max_batch > (1 << 20)) return ATMO_EINVAL;
+    *out = nullptr;
+    AtmoMlp* m = new (std::nothrow) AtmoMlp();
+    if (!m) return ATMO_ENOMEM;
+    memset(m, 0, sizeof *m);
+    m->magic = kMlpMagic;
+    m->max_batch = max_batch;
+    const size_t B = (size_t)max_batch;
+    const size_t chunks = (B + kChunk - 1) / kChunk;
+    bool ok = alloc(&m->params, ATMO_MLP_PARAMS) == hipSuccess &&
+              alloc(&m->grads, ATMO_MLP_PARAMS) == hipSuccess &&
+              alloc(&m->adam_m, ATMO_MLP_PARAMS) == hipSuccess &&
+              alloc(&m->adam_v, ATMO_MLP_PARAMS) == hipSuccess &&
+              alloc(&m->xbuf, B * 5) == hipSuccess &&
+              alloc(&m->partial, chunks * (64 * 64 + 64)) == hipSuccess &&
+              alloc(&m->loss, B * 6 / 256 + 2) == hipSuccess &&
+              alloc(&m->frag, (size_t)kFragments * 64 * 4) == hipSuccess &&
+              alloc(&m->bias, (size_t)64 * 3 + 16) == hipSuccess;
+    for (int l = 0; l < 4 && ok; ++l) {
+        ok = alloc(&m->z[l], B * kLayers[l].out) == hipSuccess &&
+             alloc(&m->dz[l], B * kLayers[l].out) == hipSuccess;
+        if (ok && l < 3) ok = alloc(&m->a[l], B * 64) == hipSuccess;
+    }
+    if (ok) ok = hipMemset(m->adam_m, 0, ATMO_MLP_PARAMS * 4) == hipSuccess &&
+                 hipMemset(m->adam_v, 0, ATMO_MLP_PARAMS * 4) == hipSuccess &&
+                 hipMemset(m->params, 0, ATMO_MLP_PARAMS * 4) == hipSuccess &&
+                 hipDeviceSynchronize() == hipSuccess;
+    if (!ok) {
+        free_mlp(m);
+        delete m;
+        return ATMO_ENOMEM;
+    }
+    m->transform.altitude_max_m = 100000.0f;
+    m->transform.distance_min_m = 1.0f;
+    m->transform.distance_max_m = 1.0e6f;
+    for (int q = 0; q < 6; ++q) m->transform.out_scale[q] = 1.0f;
+    *out = m;
+    return ATMO_OK;
+}
+
+int32_t atmo_mlp_destroy(AtmoMlp* m) {
+    if (!mlp_ok(m)) return ATMO_EHANDLE;
+    (void)hipDeviceSynchronize();
+    free_mlp(m);
+    m->magic = 0;
+    delete m;
+    return ATMO_OK;
+}
+
+const char* atmo_mlp_last_error(const AtmoMlp* m) {
+    return mlp_ok(m) ? m->error : "invalid surrogate handle";
+}
+
+int32_t atmo_mlp_set_params(AtmoMlp* m, const float* host_params, int64_t count) {
+    if (!mlp_ok(m)) return ATMO_EHANDLE;
+    if (!host_params || count != ATMO_MLP_PARAMS) return ATMO_ESHAPE;
+    if (hipMemcpy(m->params, host_params, count * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(m->adam_m, 0, count * 4) != hipSuccess ||
+        hipMemset(m->adam_v, 0, count * 4) != hipSuccess)
+        return ATMO_EHIP;
+    hipLaunchKernelGGL(k_pack_fragments, dim3(1), dim3(64), 0, 0, m->params, m->frag, m->bias);
+    if (status(m, "atmo_mlp_set_params") != ATMO_OK) return ATMO_EHIP;
+    return hipDeviceSynchronize() == hipSuccess ? ATMO_OK : ATMO_EHIP;
+}
+
+int32_t atmo_mlp_get_params(AtmoMlp* m, float* host_params, int64_t count) {
+    if (!mlp_ok(m)) return ATMO_EHANDLE;
+    if (!host_params || count != ATMO_MLP_PARAMS) return ATMO_ESHAPE;
+    return hipMemcpy(host_params, m->params, count * 4, hipMemcpyDeviceToHost) == hipSuccess
+               ? ATMO_OK
+               : ATMO_EHIP;
+}
+
+int32_t atmo_mlp_set_transform(AtmoMlp* m, const AtmoSurrogateTransform* t) {
+    if (!mlp_ok(m)) return ATMO_EHANDLE;
+    if (!t || !(t->altitude_max_m > 0.0f) || !(t->distance_min_m > 0.0f) ||
+        !(t->distance_max_m > t->distance_min_m))
+        return ATMO_EINVAL;
+    m->transform = *t;
+    return ATMO_OK;
+}
+
+int32_t atmo_mlp_forward(AtmoMlp* m, const float* x, int32_t B, float* y, void* stream) {
+    if (!mlp_ok(m)) return ATMO_EHANDLE;
+    if (!x || !y || B <= 0 || B > m->max_batch) return ATMO_ESHAPE;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for((long)B * 5, 256)), dim3(256), 0, s, B, 5, nullptr,
+                       x, m->xbuf);
+    forward_layers(m, B, s);
+    if (hipMemcpyAsync(y, m->z[3], (size_t)B * 6 * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return ATMO_EHIP;
+    return status(m, "atmo_mlp_forward");
+}
+
+int32_t atmo_mlp_gradients(AtmoMlp* m, const float* x, const float* target, const int32_t* index,
+                           int32_t B, float* grad_out, float* loss_out, void* stream) {
+    if (!mlp_ok(m)) return ATMO_EHANDLE;
+    if (!x || !target || B <= 0 || B > m->max_batch) return ATMO_ESHAPE;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for((long)B * 5, 256)), dim3(256), 0, s, B, 5, index, x,
+                       m->xbuf);
+    // the targets are gathered into dz[3] first and replaced by the gradient
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for((long)B * 6, 256)), dim3(256), 0, s, B, 6, index,
+                       target, m->dz[3]);
+    forward_layers(m, B, s);
+    const int lblocks = (int)blocks_for((long)B * 6, 256);
+    hipLaunchKernelGGL(k_mse, dim3(lblocks), dim3(256), 0, s, B, m->z[3], m->dz[3], m->dz[3], m->loss);
+    hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(256), 0, s, lblocks, (long)B * 6, m->loss);
+    backward_layers(m, B, s);
+    if (grad_out &&
+        hipMemcpyAsync(grad_out, m->grads, ATMO_MLP_PARAMS * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return ATMO_EHIP;
+    if (loss_out && hipMemcpyAsync(loss_out, m->loss, 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return ATMO_EHIP;
+    return status(m, "atmo_mlp_gradients");
+}
+
+int32_t atmo_mlp_train_step(AtmoMlp* m, const float* x, const float* target, const int32_t* index,
+                            int32_t B, float lr, int32_t step, float* loss_out, void* stream) {
+    if (!mlp_ok(m)) return ATMO_EHANDLE;
+    if (step < 1) return ATMO_EINVAL;
+    int32_t st = atmo_mlp_gradients(m, x, target, index, B, nullptr, loss_out, stream);
+    if (st != ATMO_OK) return st;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float bc1 = (float)(1.0 - pow(0.9, step)), bc2 = (float)(1.0 - pow(0.999, step));
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(ATMO_MLP_PARAMS, 256)), dim3(256), 0, (hipStream_t)stream,
+                       ATMO_MLP_PARAMS, m->params, m->grads, m->adam_m, m->adam_v, lr, b1, b2, 1.0e-8f,
+                       bc1, bc2);
+    return status(m, "atmo_mlp_train_step");
+}
+
+int32_t atmo_mlp_pack(AtmoMlp* m, void* stream) {
+    if (!mlp_ok(m)) return ATMO_EHANDLE;
+    hipLaunchKernelGGL(k_pack_fragments, dim3(1), dim3(64), 0, (hipStream_t)stream, m->params, m->frag,
+                       m->bias);
+    return status(m, "atmo_mlp_pack");
+}
+
+int32_t atmo_mlp_infer(AtmoMlp* m, const float* x, int64_t n, float* y, void* stream) {
+    if (!mlp_ok(m)) return ATMO_EHANDLE;
+    if (!x || !y || n <= 0) return ATMO_ESHAPE;
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, n, x,
+                       m->frag, m->bias, y);
+    return status(m, "atmo_mlp_infer");
+}
+
+int32_t atmo_mlp_encode(AtmoMlp* m, const float* raw, int64_t n, float* x, void* stream) {
+    if (!mlp_ok(m)) return ATMO_EHANDLE;
+    if (!raw || !x || n <= 0) return ATMO_ESHAPE;
+    hipLaunchKernelGGL(k_encode, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, n, raw,
+                       m->transform, x);
+    return status(m, "atmo_mlp_encode");
+}
+
+int32_t atmo_surrogate_labels(AtmoHandle* h, const float* raw, int64_t n, float bottom,
+                              float sun_angular_radius, float* labels, void* stream) {
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
+    if (!raw || !labels || n <= 0) {
+        atmo_set_error(h, "atmo_surrogate_labels: bad arguments");
+        return ATMO_EINVAL;
+    }
+    hipLaunchKernelGGL(k_labels, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, h->k, (long)n,
+                       raw, bottom, sun_angular_radius, h->tlut, h->mslut, labels);
+    return atmo_launch_status(h, "atmo_surrogate_labels");
+}
+
+int32_t atmo_composite_surrogate(AtmoHandle* h, AtmoMlp* m, const float* scene_hdr,
+                                 const float* space_hdr, const int32_t* surface_id,
+                                 const float* position_view, float* hdr, float bottom, float cam_alt,
+                                 float sky_alt, const float sun[3], const float irradiance[3],
+                                 const float disk[3], float ar, const float right[3], const float up[3],
+                                 const float forward[3], float thf, float c0, float c1, void* stream) {
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
+    if (!mlp_ok(m)) {
+        atmo_set_error(h, "atmo_composite_surrogate: invalid surrogate handle");
+        return ATMO_EINVAL;
+    }
+    if (!scene_hdr || !space_hdr || !surface_id || !position_view || !hdr || !sun || !irradiance ||
+        !disk || !right || !up || !forward) {
+        atmo_set_error(h, "atmo_composite_surrogate: NULL argument");
+        return ATMO_EINVAL;
+    }
+    SurrogateArgs a;
+    a.scene = scene_hdr;
+    a.space = space_hdr;
+    a.sid = surface_id;
+    a.pos = position_view;
+    a.hdr = hdr;
+    a.tlut = h->tlut;
+    a.mslut = h->mslut;
+    a.svlut = h->svlut;
+    a.width = h->width;
+    a.height = h->height;
+    a.bottom = bottom;
+    a.cam_alt = cam_alt;
+    a.sky_alt = sky_alt;
+    a.sun = mk3(sun);
+    a.irradiance = mk3(irradiance);
+    a.disk = mk3(disk);
+    a.ar = ar;
+    a.right = mk3(right);
+    a.up = mk3(up);
+    a.forward = mk3(forward);
+    a.thf = thf;
+    a.c0 = c0;
+    a.c1 = c1;
+    a.frag = m->frag;
+    a.bias = m->bias;
+    a.t = m->transform;
+    const long npix = (long)h->width * h->height;
+    hipLaunchKernelGGL(k_composite_surrogate, dim3(blocks_for(npix, 256)), dim3(256), 0,
+                       (hipStream_t)stream, h->k, a);
+    return atmo_launch_status(h, "atmo_composite_surrogate");
+}
+
+}  // extern "C"

@@ -210,6 +210,24 @@ class AtmosphereRenderer:
             f3(right), f3(view_up), f3(forward), tangent_half_fov, star_contrast_start,
             star_contrast_end, int(diagnostic_view), self._stream()), self._handle)
 
+    def _composite_surrogate(self, surrogate, scene_hdr, space_hdr, surface_id, position_view,
+                             hdr, bottom_radius, camera_altitude, sky_lut_camera_altitude,
+                             sun_local, solar_irradiance, sun_disk_radiance, sun_angular_radius,
+                             right, view_up, forward, tangent_half_fov, star_contrast_start,
+                             star_contrast_end) -> None:
+        """COMPOSITE view with the learned surface branch (atmo_composite_surrogate)."""
+        scene = self._frame(scene_hdr, "scene_hdr")
+        space = self._frame(space_hdr, "space_hdr")
+        sid = self._frame(surface_id, "surface_id", channels=False)
+        pos = self._frame(position_view, "position_view")
+        out = self._frame(hdr, "hdr")
+        check(self._lib.atmo_composite_surrogate(
+            self._handle, surrogate.handle, scene.data_ptr(), space.data_ptr(), sid.data_ptr(),
+            pos.data_ptr(), out.data_ptr(), bottom_radius, camera_altitude,
+            sky_lut_camera_altitude, f3(sun_local), f3(solar_irradiance), f3(sun_disk_radiance),
+            sun_angular_radius, f3(right), f3(view_up), f3(forward), tangent_half_fov,
+            star_contrast_start, star_contrast_end, self._stream()), self._handle)
+
     # ----------------------------------------------------------- host policy
 
     def set_luts_frozen(self, frozen: bool) -> None:
@@ -286,14 +304,25 @@ class AtmosphereRenderer:
                   tangent_half_fov: float, star_contrast_start: float,
                   star_contrast_end: float,
                   diagnostic_view: AtmosphereDiagnosticView = AtmosphereDiagnosticView.COMPOSITE,
-                  ) -> None:
+                  surrogate=None) -> None:
         """Per-pixel composite into `hdr` (renderer.py:2189-2238).  Sky pixels
-        sample sky-view at its snapshot altitude."""
+        sample sky-view at its snapshot altitude.  With a trained
+        `surrogate` (surrogate.SurrogateMLP) the COMPOSITE view takes the
+        surface in-scatter and transmittance from the network for in-domain
+        pixels."""
         right, view_up, forward = view_basis
         camera_altitude = max(float(camera_radius_m) - float(planet_radius_m), 0.0)
         sky_lut_camera_altitude = camera_altitude
         if self.sky_snapshot_altitude_m is not None:
             sky_lut_camera_altitude = self.sky_snapshot_altitude_m
+        if surrogate is not None and int(diagnostic_view) == int(AtmosphereDiagnosticView.COMPOSITE):
+            self._composite_surrogate(surrogate, scene_hdr, space_hdr, surface_id, position_view,
+                                      hdr, float(planet_radius_m), camera_altitude,
+                                      sky_lut_camera_altitude, sun_local, solar_irradiance,
+                                      sun_disk_radiance, math.radians(sun_angular_radius_degrees),
+                                      right, view_up, forward, float(tangent_half_fov),
+                                      float(star_contrast_start), float(star_contrast_end))
+            return
         self._composite(scene_hdr, space_hdr, surface_id, position_view, hdr,
                         float(planet_radius_m), camera_altitude, sky_lut_camera_altitude,
                         sun_local, solar_irradiance, sun_disk_radiance,

@@ -1,0 +1,350 @@
+"""Learned surrogate of the composite's surface branch.
+
+The plain composite integrates, per surface pixel, the direct limb
+in-scatter (renderer.py:1667-1786) and the two-sample segment transmittance
+(renderer.py:1603-1665).  For a ray from the camera on the local vertical,
+both depend only on (camera altitude h, view-zenith cosine mu, sun-zenith
+cosine mu_s, relative azimuth phi, segment length d), and the in-scatter is
+linear in the solar irradiance (the multi-scattering LUT is too), so a
+5 -> 64 -> 64 -> 64 -> 6 SiLU MLP learns (L / E, T) once per atmosphere.
+
+Everything numeric runs in libatmo.so: labels from the GPU integrator
+(atmo_surrogate_labels), FP32 forward / backward / Adam training kernels,
+BF16 MFMA inference.  This module drives them and handles the target
+transform (log of in-scatter and transmittance, standardised per output)
+and the weight file (raw float32 parameters + JSON metadata).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import json
+import math
+import time
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+from . import _capi
+from ._capi import AtmoSurrogateTransform, library
+
+ALTITUDE_MAX_M = 100_000.0
+DISTANCE_RANGE_M = (1.0, 1.0e6)
+EPS_INSCATTER = 1.0e-8
+EPS_TRANSMITTANCE = 1.0e-6
+SEED_BASE = 20261203
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def sample_inputs(n: int, seed: int) -> np.ndarray:
+    """(n, 5) float32 raw samples: h ~ U[0, 100 km], mu ~ U[-1, 1],
+    mu_s ~ U[-1, 1], phi ~ U[0, pi], d ~ logU[1 m, 1000 km]."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    lo, hi = DISTANCE_RANGE_M
+    raw = np.empty((n, 5), dtype=np.float64)
+    raw[:, 0] = g.uniform(0.0, ALTITUDE_MAX_M, n)
+    raw[:, 1] = g.uniform(-1.0, 1.0, n)
+    raw[:, 2] = g.uniform(-1.0, 1.0, n)
+    raw[:, 3] = g.uniform(0.0, math.pi, n)
+    raw[:, 4] = np.exp(g.uniform(math.log(lo), math.log(hi), n))
+    return raw.astype(np.float32)
+
+
+def targets_from_labels(labels: np.ndarray) -> np.ndarray:
+    """log(L + eps) for the in-scatter columns, log(T + eps) for the
+    transmittance columns."""
+    labels = np.asarray(labels, dtype=np.float64)
+    out = np.empty_like(labels)
+    out[:, :3] = np.log(np.maximum(labels[:, :3], 0.0) + EPS_INSCATTER)
+    out[:, 3:] = np.log(np.clip(labels[:, 3:], 0.0, 1.0) + EPS_TRANSMITTANCE)
+    return out
+
+
+def decode_outputs(raw: np.ndarray, mean: np.ndarray, scale: np.ndarray):
+    """Network outputs -> (L per unit irradiance, T), the inverse of
+    targets_from_labels after standardisation (host mirror of the kernel's
+    decode)."""
+    y = np.asarray(raw, dtype=np.float64) * scale + mean
+    L = np.maximum(np.exp(y[:, :3]) - EPS_INSCATTER, 0.0)
+    T = np.clip(np.exp(y[:, 3:]) - EPS_TRANSMITTANCE, 0.0, 1.0)
+    return L, T
+
+
+def encode_host(raw: np.ndarray) -> np.ndarray:
+    """Host mirror of the kernel's input normalisation (float64)."""
+    raw = np.asarray(raw, dtype=np.float64)
+    lo, hi = math.log(DISTANCE_RANGE_M[0]), math.log(DISTANCE_RANGE_M[1])
+    x = np.empty_like(raw)
+    x[:, 0] = raw[:, 0] * (2.0 / ALTITUDE_MAX_M) - 1.0
+    x[:, 1] = raw[:, 1]
+    x[:, 2] = raw[:, 2]
+    x[:, 3] = raw[:, 3] * (2.0 / math.pi) - 1.0
+    x[:, 4] = (np.log(raw[:, 4]) - lo) * (2.0 / (hi - lo)) - 1.0
+    return x
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-uniform weights, zero biases, in the flat layout of include/atmo.h."""
+    g = np.random.default_rng(seed)
+    parts = []
+    for fan_in, fan_out in ((5, 64), (64, 64), (64, 64), (64, 6)):
+        bound = math.sqrt(6.0 / fan_in)
+        parts.append(g.uniform(-bound, bound, fan_in * fan_out))
+        parts.append(np.zeros(fan_out))
+    return np.concatenate(parts).astype(np.float32)
+
+
+@dataclass
+class TrainingReport:
+    samples: int
+    batch: int
+    epochs: int
+    steps: int
+    lr: float
+    seconds: float
+    label_seconds: float
+    losses: list = field(default_factory=list)
+
+
+class SurrogateMLP:
+    """Device-resident network (AtmoMlp handle of libatmo.so)."""
+
+    def __init__(self, max_batch: int = 65536, device=None):
+        torch = _torch()
+        self.device = torch.device(device if device is not None else "cuda")
+        self._lib = library()
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            status = self._lib.atmo_mlp_create(int(max_batch), ctypes.byref(handle))
+        if status != _capi.ATMO_OK:
+            raise _capi.AtmoError(f"atmo_mlp_create failed with status {status}")
+        self._handle = handle
+        self.max_batch = int(max_batch)
+        self.out_mean = np.zeros(6)
+        self.out_scale = np.ones(6)
+        self.metadata: dict = {}
+        self.set_params(init_params(0))
+        self._push_transform()
+
+    # ---------------------------------------------------------------- basics
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._handle
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None) is not None and self._handle.value:
+            self._lib.atmo_mlp_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, status: int) -> None:
+        if status != _capi.ATMO_OK:
+            message = self._lib.atmo_mlp_last_error(self._handle).decode(errors="replace")
+            raise (ValueError if status in (_capi.ATMO_EINVAL, _capi.ATMO_ESHAPE)
+                   else _capi.AtmoError)(f"surrogate status {status}: {message}")
+
+    def _stream(self) -> int:
+        return _torch().cuda.current_stream(self.device).cuda_stream
+
+    def transform(self) -> AtmoSurrogateTransform:
+        t = AtmoSurrogateTransform()
+        t.altitude_max_m = ALTITUDE_MAX_M
+        t.distance_min_m, t.distance_max_m = DISTANCE_RANGE_M
+        t.out_mean[:] = [float(v) for v in self.out_mean]
+        t.out_scale[:] = [float(v) for v in self.out_scale]
+        t.eps_inscatter = EPS_INSCATTER
+        t.eps_transmittance = EPS_TRANSMITTANCE
+        return t
+
+    def _push_transform(self) -> None:
+        t = self.transform()
+        self._check(self._lib.atmo_mlp_set_transform(self._handle, ctypes.byref(t)))
+
+    def set_params(self, params: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.shape != (_capi.MLP_PARAMS,):
+            raise ValueError(f"expected {_capi.MLP_PARAMS} parameters, got {p.shape}")
+        self._check(self._lib.atmo_mlp_set_params(
+            self._handle, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.size))
+
+    def get_params(self) -> np.ndarray:
+        _torch().cuda.synchronize(self.device)
+        p = np.zeros(_capi.MLP_PARAMS, dtype=np.float32)
+        self._check(self._lib.atmo_mlp_get_params(
+            self._handle, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.size))
+        return p
+
+    def set_output_transform(self, mean, scale) -> None:
+        self.out_mean = np.asarray(mean, dtype=np.float64)
+        self.out_scale = np.asarray(scale, dtype=np.float64)
+        self._push_transform()
+
+    # ------------------------------------------------------------- compute
+
+    def encode(self, raw):
+        torch = _torch()
+        x = torch.empty((raw.shape[0], 5), dtype=torch.float32, device=self.device)
+        self._check(self._lib.atmo_mlp_encode(self._handle, raw.data_ptr(), raw.shape[0],
+                                              x.data_ptr(), self._stream()))
+        return x
+
+    def forward(self, x):
+        torch = _torch()
+        y = torch.empty((x.shape[0], 6), dtype=torch.float32, device=self.device)
+        self._check(self._lib.atmo_mlp_forward(self._handle, x.data_ptr(), x.shape[0],
+                                               y.data_ptr(), self._stream()))
+        return y
+
+    def gradients(self, x, target, index=None, batch=None):
+        torch = _torch()
+        batch = int(batch if batch is not None else x.shape[0])
+        grad = torch.empty(_capi.MLP_PARAMS, dtype=torch.float32, device=self.device)
+        loss = torch.empty(1, dtype=torch.float32, device=self.device)
+        self._check(self._lib.atmo_mlp_gradients(
+            self._handle, x.data_ptr(), target.data_ptr(),
+            index.data_ptr() if index is not None else None, batch, grad.data_ptr(),
+            loss.data_ptr(), self._stream()))
+        return grad, loss
+
+    def train_step(self, x, target, index, batch: int, lr: float, step: int, loss_out=None):
+        self._check(self._lib.atmo_mlp_train_step(
+            self._handle, x.data_ptr(), target.data_ptr(),
+            index.data_ptr() if index is not None else None, int(batch), float(lr), int(step),
+            loss_out.data_ptr() if loss_out is not None else None, self._stream()))
+
+    def pack(self) -> None:
+        self._check(self._lib.atmo_mlp_pack(self._handle, self._stream()))
+
+    def infer(self, x):
+        """BF16 MFMA inference of raw outputs for encoded inputs."""
+        torch = _torch()
+        y = torch.empty((x.shape[0], 6), dtype=torch.float32, device=self.device)
+        self._check(self._lib.atmo_mlp_infer(self._handle, x.data_ptr(), x.shape[0],
+                                             y.data_ptr(), self._stream()))
+        return y
+
+    # ------------------------------------------------------------- file i/o
+
+    def save(self, path) -> None:
+        """<path>.bin: float32 parameters (little-endian);
+        <path>.json: output transform, domain and training metadata."""
+        path = Path(path)
+        self.get_params().astype("<f4").tofile(path.with_suffix(".bin"))
+        meta = dict(self.metadata)
+        meta.update({
+            "format": "atmo-surrogate-v1",
+            "layers": [[5, 64], [64, 64], [64, 64], [64, 6]],
+            "activation": "silu",
+            "params": _capi.MLP_PARAMS,
+            "out_mean": [float(v) for v in self.out_mean],
+            "out_scale": [float(v) for v in self.out_scale],
+            "altitude_max_m": ALTITUDE_MAX_M,
+            "distance_range_m": list(DISTANCE_RANGE_M),
+            "eps_inscatter": EPS_INSCATTER,
+            "eps_transmittance": EPS_TRANSMITTANCE,
+        })
+        path.with_suffix(".json").write_text(json.dumps(meta, indent=1))
+
+    @classmethod
+    def load(cls, path, max_batch: int = 65536, device=None) -> "SurrogateMLP":
+        path = Path(path)
+        meta = json.loads(path.with_suffix(".json").read_text())
+        if meta.get("format") != "atmo-surrogate-v1":
+            raise ValueError(f"{path}: not an atmo surrogate weight file")
+        params = np.fromfile(path.with_suffix(".bin"), dtype="<f4")
+        net = cls(max_batch=max_batch, device=device)
+        net.set_params(params)
+        net.set_output_transform(meta["out_mean"], meta["out_scale"])
+        net.metadata = meta
+        return net
+
+
+def generate_labels(renderer, raw_host: np.ndarray, planet_radius_m: float,
+                    sun_angular_radius_deg: float):
+    """Labels from the GPU integrator: builds the renderer's T- and MS-LUTs
+    for unit irradiance, then evaluates the surface branch per sample.
+    Returns (raw_dev, labels_dev) torch tensors."""
+    torch = _torch()
+    ar = float(np.float32(math.radians(sun_angular_radius_deg)))
+    renderer._build_transmittance(planet_radius_m)
+    renderer._build_multi_scattering(planet_radius_m, (1.0, 1.0, 1.0), ar)
+    raw = torch.from_numpy(np.ascontiguousarray(raw_host, dtype=np.float32)).to(renderer.device)
+    labels = torch.empty((raw.shape[0], 6), dtype=torch.float32, device=renderer.device)
+    stream = torch.cuda.current_stream(renderer.device).cuda_stream
+    _capi.check(library().atmo_surrogate_labels(renderer.handle, raw.data_ptr(), raw.shape[0],
+                                                float(planet_radius_m), math.radians(
+                                                    sun_angular_radius_deg),
+                                                labels.data_ptr(), stream), renderer.handle)
+    return raw, labels
+
+
+def train(renderer, planet_radius_m: float, sun_angular_radius_deg: float,
+          samples: int = 1 << 22, batch: int = 65536, epochs: int = 100, lr: float = 1.0e-3,
+          seed: int = SEED_BASE + 3, log=None) -> tuple[SurrogateMLP, TrainingReport]:
+    """Label `samples` seeded inputs with the GPU integrator and train the
+    MLP with Adam (constant lr) for `epochs` passes in shuffled batches."""
+    torch = _torch()
+    device = renderer.device
+    t0 = time.perf_counter()
+    raw_host = sample_inputs(samples, seed)
+    raw, labels = generate_labels(renderer, raw_host, planet_radius_m, sun_angular_radius_deg)
+    torch.cuda.synchronize(device)
+    label_s = time.perf_counter() - t0
+    targets = targets_from_labels(labels.cpu().numpy())
+    mean, scale = targets.mean(axis=0), targets.std(axis=0)
+    scale = np.where(scale > 1e-6, scale, 1.0)
+    t_std = torch.from_numpy(((targets - mean) / scale).astype(np.float32)).to(device)
+    net = SurrogateMLP(max_batch=batch, device=device)
+    net.set_output_transform(mean, scale)
+    net.set_params(init_params(seed))
+    x = net.encode(raw)
+    g = np.random.Generator(np.random.PCG64(seed + 1))
+    steps_per_epoch = samples // batch
+    loss = torch.zeros(1, dtype=torch.float32, device=device)
+    report = TrainingReport(samples, batch, epochs, 0, lr, 0.0, label_s)
+    step = 0
+    t1 = time.perf_counter()
+    for epoch in range(epochs):
+        perm = torch.from_numpy(g.permutation(samples).astype(np.int32)).to(device)
+        for b in range(steps_per_epoch):
+            step += 1
+            net.train_step(x, t_std, perm[b * batch:(b + 1) * batch], batch, lr, step, loss)
+        if epoch % 10 == 9 or epoch == epochs - 1:
+            value = float(loss.item())
+            report.losses.append((epoch + 1, value))
+            if log:
+                log(f"surrogate epoch {epoch + 1}/{epochs}: loss {value:.3e}")
+    net.pack()
+    torch.cuda.synchronize(device)
+    report.steps = step
+    report.seconds = time.perf_counter() - t1
+    net.metadata.update({"samples": samples, "batch": batch, "epochs": epochs, "lr": lr,
+                         "seed": seed, "planet_radius_m": planet_radius_m,
+                         "sun_angular_radius_deg": sun_angular_radius_deg,
+                         "final_loss": report.losses[-1][1] if report.losses else None})
+    return net, report
+
+
+__all__ = [
+    "SurrogateMLP",
+    "TrainingReport",
+    "decode_outputs",
+    "encode_host",
+    "generate_labels",
+    "init_params",
+    "sample_inputs",
+    "targets_from_labels",
+    "train",
+]

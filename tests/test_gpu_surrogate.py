@@ -1,0 +1,224 @@
+"""GPU checks of the surrogate: MFMA inference and FP32 training kernels
+against the float64 MLP, labels against the float32 CPU restatement of the
+integrator, training progress, and the out-of-domain fallback of the
+surrogate composite."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import mlp_fp64
+from parity import PLAIN_TOLERANCE, rel_error, summary
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def device():
+    import torch
+
+    assert torch.cuda.is_available()
+    return torch.device("cuda:0")
+
+
+@pytest.fixture(scope="module")
+def sur(pkg):
+    import importlib
+
+    return importlib.import_module(pkg.__name__ + ".surrogate")
+
+
+def _dev(a, device, dtype=np.float32):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=dtype)).to(device)
+
+
+def bf16(a):
+    """Round to bfloat16 (nearest even), returned as float64."""
+    u = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def quantized_forward(params, x):
+    """Float64 forward with the inference kernel's quantisation: bf16
+    weights and layer inputs, fp32 biases."""
+    layers = mlp_fp64.unpack(params)
+    a = bf16(x)
+    for i, (w, b) in enumerate(layers):
+        z = a @ bf16(w).T + b
+        a = bf16(mlp_fp64.silu(z)) if i < 3 else z
+    return a
+
+
+def _random_params(seed, spread=0.05):
+    g = np.random.default_rng(seed)
+    return (mlp_fp64.init_params(seed) + g.normal(0, spread, mlp_fp64.PARAM_COUNT)).astype(np.float32)
+
+
+def test_mfma_inference_matches_quantised_fp64(sur, device):
+    """v_mfma_f32_16x16x16bf16 lane layout and the fused 4-layer chain: the
+    BF16 inference equals a float64 forward with the same quantisation."""
+    net = sur.SurrogateMLP(max_batch=4096, device=device)
+    params = _random_params(1)
+    net.set_params(params)
+    x = np.random.default_rng(2).uniform(-1, 1, (3000, 5)).astype(np.float32)
+    y = net.infer(_dev(x, device)).cpu().numpy()
+    want_q = quantized_forward(params.astype(np.float64), x)
+    want = mlp_fp64.forward(params.astype(np.float64), x)[0]
+    scale = np.abs(want).max()
+    err_q = np.abs(y - want_q).max() / scale
+    err = np.abs(y - want).max() / scale
+    print(f"MFMA inference: max err vs quantised fp64 {err_q:.2e}, vs fp64 {err:.2e}", flush=True)
+    assert err_q < 5e-3
+    assert err < 5e-2
+    net.close()
+
+
+def test_fp32_forward_matches_fp64(sur, device):
+    net = sur.SurrogateMLP(max_batch=4096, device=device)
+    params = _random_params(3)
+    net.set_params(params)
+    x = np.random.default_rng(4).uniform(-1, 1, (4096, 5)).astype(np.float32)
+    y = net.forward(_dev(x, device)).cpu().numpy()
+    want = mlp_fp64.forward(params.astype(np.float64), x)[0]
+    np.testing.assert_allclose(y, want, rtol=1e-4, atol=1e-5 * np.abs(want).max())
+    net.close()
+
+
+def test_gradients_match_fp64(sur, device):
+    """FP32 backward kernels (data and two-stage weight reduction) against
+    the analytic float64 gradient, itself pinned by finite differences."""
+    net = sur.SurrogateMLP(max_batch=8192, device=device)
+    params = _random_params(5)
+    net.set_params(params)
+    g = np.random.default_rng(6)
+    x = g.uniform(-1, 1, (8192, 5)).astype(np.float32)
+    t = g.normal(0, 1, (8192, 6)).astype(np.float32)
+    idx = g.permutation(8192).astype(np.int32)[:5000]
+    grad, loss = net.gradients(_dev(x, device), _dev(t, device), _dev(idx, device, np.int32),
+                               batch=5000)
+    want, want_loss = mlp_fp64.gradients(params.astype(np.float64), x[idx], t[idx])
+    got = grad.cpu().numpy().astype(np.float64)
+    rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+    worst = np.abs(got - want).max() / np.abs(want).max()
+    print(f"gradients: norm-rel {rel:.2e}, max {worst:.2e}", flush=True)
+    assert rel < 1e-4 and worst < 1e-3
+    assert float(loss.item()) == pytest.approx(want_loss, rel=1e-4)
+    net.close()
+
+
+def test_adam_step_matches_fp64(sur, device):
+    import torch
+
+    net = sur.SurrogateMLP(max_batch=2048, device=device)
+    params = _random_params(7)
+    net.set_params(params)
+    g = np.random.default_rng(8)
+    x = g.uniform(-1, 1, (2048, 5)).astype(np.float32)
+    t = g.normal(0, 1, (2048, 6)).astype(np.float32)
+    xd, td = _dev(x, device), _dev(t, device)
+    p = params.astype(np.float64)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for step in (1, 2, 3):
+        net.train_step(xd, td, None, 2048, 1e-3, step)
+        grad, _ = mlp_fp64.gradients(p, x, t)
+        p, m, v = mlp_fp64.adam_step(p, grad, m, v, step, lr=1e-3)
+    torch.cuda.synchronize(device)
+    got = net.get_params().astype(np.float64)
+    moved = np.abs(p - params.astype(np.float64))
+    diff = np.abs(got - p)
+    # Early Adam updates are ~lr * sign(g); they agree except where |g| is
+    # near eps, where float32 gradient rounding decides the update.
+    assert np.percentile(diff, 99) < 1e-5
+    assert diff.max() <= 3.0e-3 + 1e-6
+    assert moved.max() > 1e-3
+    net.close()
+
+
+def test_labels_match_cpu_integrator(pkg, scene, sur, ref32, device):
+    """atmo_surrogate_labels is the plain integrator: parity with the
+    float32 restatement of the composite's surface branch."""
+    import torch
+
+    cfg = scene.full_frame_config()
+    R = scene.EARTH_RADIUS_M
+    r = pkg.AtmosphereRenderer(16, 16, cfg, device)
+    raw = sur.sample_inputs(20_000, 11)
+    raw_dev, labels = sur.generate_labels(r, raw, R, scene.SUN_ANGULAR_RADIUS_DEG)
+    torch.cuda.synchronize(device)
+    got = labels.cpu().numpy()
+    tlut = ref32.build_transmittance(cfg, R)
+    ar = float(np.float32(math.radians(scene.SUN_ANGULAR_RADIUS_DEG)))
+    mslut = ref32.build_multi_scattering(cfg, R, (1.0, 1.0, 1.0), ar, tlut)
+    # the kernel's canonical vectors, in float32 arithmetic with correctly
+    # rounded cos / sin (sample_geometry in surrogate.hip)
+    f = np.float32
+    mu, mus, phi = raw[:, 1], raw[:, 2], raw[:, 3]
+    sr = np.sqrt(np.maximum(f(1) - mu * mu, f(0)))
+    ss = np.sqrt(np.maximum(f(1) - mus * mus, f(0)))
+    cphi = np.cos(phi.astype(np.float64)).astype(f)
+    sphi = np.sin(phi.astype(np.float64)).astype(f)
+    rays = np.column_stack([sr, mu, np.zeros_like(mu)]).astype(f)
+    suns = np.column_stack([ss * cphi, mus, ss * sphi]).astype(f)
+    L, T = ref32.surface_path(cfg, raw[:, 0], rays, raw[:, 4], suns, (1.0, 1.0, 1.0), R,
+                              math.radians(scene.SUN_ANGULAR_RADIUS_DEG), tlut, mslut)
+    for name, a, b in (("inscatter", got[:, :3], L), ("transmittance", got[:, 3:], T)):
+        s = summary(rel_error(a, b))
+        print(f"labels {name}: max {s['max']:.2e} p99 {s['p99']:.2e}", flush=True)
+        assert s["p99"] <= PLAIN_TOLERANCE
+        assert s["max"] <= 1e-3
+    r.close()
+
+
+def test_training_reduces_loss_and_weights_round_trip(pkg, scene, sur, device, tmp_path):
+    r = pkg.AtmosphereRenderer(16, 16, scene.full_frame_config(), device)
+    net, report = sur.train(r, scene.EARTH_RADIUS_M, scene.SUN_ANGULAR_RADIUS_DEG,
+                            samples=1 << 16, batch=4096, epochs=20)
+    first, last = report.losses[0][1], report.losses[-1][1]
+    print(f"training: loss {first:.3e} -> {last:.3e} in {report.seconds:.2f} s", flush=True)
+    assert last < 0.5 * first and last < 0.2
+    path = tmp_path / "weights"
+    net.save(path)
+    again = sur.SurrogateMLP.load(path, max_batch=4096, device=device)
+    np.testing.assert_array_equal(again.get_params(), net.get_params())
+    np.testing.assert_allclose(again.out_mean, net.out_mean)
+    x = _dev(np.random.default_rng(0).uniform(-1, 1, (1000, 5)), device)
+    np.testing.assert_array_equal(again.infer(x).cpu().numpy(), net.infer(x).cpu().numpy())
+    net.close()
+    again.close()
+    r.close()
+
+
+def test_surrogate_composite_falls_back_outside_domain(pkg, scene, sur, device):
+    """Pixels outside the training domain (segments beyond 1000 km, camera
+    above 100 km) and sky pixels are computed by the plain integrator:
+    identical to the plain composite."""
+    import torch
+
+    net = sur.SurrogateMLP(max_batch=1024, device=device)
+    net.set_params(_random_params(9))
+    for altitude, dist in ((150_000.0, (10.0, 5.0e5)), (20_000.0, (1.5e6, 3.0e6))):
+        inputs = scene.make_frame(width=128, height=72, config_index=1, distance_range=dist,
+                                  camera_altitude_range=(altitude, altitude))
+        r = pkg.AtmosphereRenderer(inputs.width, inputs.height, inputs.config, device)
+        light = inputs.lighting
+        pos, sid = _dev(inputs.position_view, device), _dev(inputs.surface_id, device, np.int32)
+        scene_hdr, space = _dev(inputs.scene_hdr, device), _dev(inputs.space_hdr, device)
+        r.update(inputs.planet_radius_m, inputs.camera_radius_m, inputs.sun_local,
+                 light.solar_irradiance, light.sun_angular_radius_degrees)
+        outs = []
+        for s in (None, net):
+            hdr = torch.full_like(scene_hdr, float("nan"))
+            r.composite(scene_hdr, space, sid, pos, hdr, inputs.planet_radius_m,
+                        inputs.camera_radius_m, inputs.sun_local, light.solar_irradiance,
+                        light.sun_disk_radiance, light.sun_angular_radius_degrees,
+                        inputs.view_basis, inputs.tangent_half_fov, *inputs.star_contrast,
+                        surrogate=s)
+            outs.append(hdr.cpu().numpy())
+        np.testing.assert_array_equal(outs[0], outs[1])
+        r.close()
+    net.close()

@@ -1,0 +1,249 @@
+"""Host-side behaviour that needs no GPU: the configuration contract, the
+lighting state, the C-ABI library's exports and parameter validation, the
+LUT-cache policy of the facade, and the synthetic workload generator."""
+
+import ctypes
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+REPO = Path(__file__).resolve().parents[1]
+
+
+# ------------------------------------------------------------ configuration
+
+def test_config_defaults_match_reference(pkg):
+    """Defaults of atmosphere/config.py:33-59."""
+    c = pkg.AtmosphereConfig()
+    assert c.top_altitude_m == 100_000.0
+    assert c.mie_extinction_per_m == (4.44e-6,) * 3
+    assert (c.sky_view_lut_width, c.sky_view_lut_height) == (192, 256)
+    assert (c.aerial_lut_width, c.aerial_lut_height, c.aerial_lut_depth) == (32, 18, 32)
+    assert (c.multi_scattering_directions, c.multi_scattering_steps) == (16, 12)
+    assert (c.aerial_horizon_raymarch_steps, c.aerial_terminator_substeps) == (12, 8)
+
+
+@pytest.mark.parametrize(
+    "changes, message",
+    [
+        ({"ground_albedo": (0.1, 1.5, 0.1)}, "atmosphere.ground_albedo must remain in [0, 1]"),
+        ({"rayleigh_scattering_per_m": (1e-6, -1e-6, 1e-6)},
+         "atmosphere.rayleigh_scattering_per_m must contain three non-negative values"),
+        ({"top_altitude_m": 0.0}, "atmosphere.top_altitude_m must be positive and finite"),
+        ({"mie_scale_height_m": math.inf}, "atmosphere.mie_scale_height_m must be positive and finite"),
+        ({"absorption_peak_altitude_m": 200_000.0},
+         "atmosphere absorption peak must lie inside the shell"),
+        ({"mie_phase_g": 0.99}, "atmosphere.mie_phase_g must be in (-0.99, 0.99)"),
+        ({"mie_scattering_per_m": (5e-6, 5e-6, 5e-6)}, "Mie scattering cannot exceed Mie extinction"),
+        ({"transmittance_lut_width": 8}, "transmittance LUT width must be in 16..2048"),
+        ({"aerial_lut_depth": 4}, "aerial LUT depth must be in 8..256"),
+        ({"multi_scattering_directions": 129}, "multi_scattering_directions must be in 4..128"),
+        ({"aerial_terminator_substeps": 1}, "aerial_terminator_substeps must be in 2..16"),
+    ],
+)
+def test_config_validation_messages(pkg, changes, message):
+    """Python facade and C-ABI reject the same configurations with the
+    reference's messages (config.py:61-128)."""
+    with pytest.raises(ValueError) as err:
+        pkg.AtmosphereConfig().replace(**changes)
+    assert str(err.value) == message
+    # Same check through atmo_validate_params, bypassing the dataclass.
+    capi = __import__(pkg.__name__ + "._capi", fromlist=["_capi"])
+    params = pkg.AtmosphereConfig().to_params()
+    for name, value in changes.items():
+        if isinstance(value, tuple):
+            getattr(params, name)[:] = list(value)
+        else:
+            setattr(params, name, value)
+    buf = ctypes.create_string_buffer(256)
+    status = capi.library().atmo_validate_params(ctypes.byref(params), buf, 256)
+    assert status == capi.ATMO_EINVAL
+    assert buf.value.decode() == message
+
+
+def test_from_mapping_round_trip(pkg):
+    """config.py:130-259: absent keys keep defaults, RGB lists are checked."""
+    c = pkg.AtmosphereConfig.from_mapping({"top_altitude_m": 80_000, "sky_view_steps": 30,
+                                           "ground_albedo": [0.2, 0.2, 0.3]})
+    assert c.top_altitude_m == 80_000.0 and c.sky_view_steps == 30
+    assert c.ground_albedo == (0.2, 0.2, 0.3)
+    assert c.mie_phase_g == 0.8
+    with pytest.raises(ValueError, match="must contain three values"):
+        pkg.AtmosphereConfig.from_mapping({"ground_albedo": [0.1, 0.2]})
+
+
+def test_params_struct_mirrors_config(pkg):
+    c = pkg.AtmosphereConfig(mie_phase_g=0.7, sky_view_steps=30)
+    p = c.to_params()
+    assert p.mie_phase_g == 0.7 and p.sky_view_steps == 30
+    assert list(p.rayleigh_scattering_per_m) == list(c.rayleigh_scattering_per_m)
+    capi = __import__(pkg.__name__ + "._capi", fromlist=["_capi"])
+    assert capi.library().atmo_validate_params(ctypes.byref(p), None, 0) == capi.ATMO_OK
+
+
+# ----------------------------------------------------------------- lighting
+
+def test_solar_disk_radiance_integrates_to_irradiance(pkg):
+    """tests/test_m0_geometry.py:87-107"""
+    state = pkg.LightingState(np.array([0.0, 1.0, 0.0]), 0.266, (4.0, 3.9, 3.7))
+    integrated = np.asarray(state.sun_disk_radiance) * state.sun_projected_solid_angle_sr
+    np.testing.assert_allclose(integrated, state.solar_irradiance, rtol=1e-12)
+    smaller = pkg.LightingState(np.array([0.0, 1.0, 0.0]), 0.133, state.solar_irradiance)
+    assert smaller.sun_disk_radiance[0] > state.sun_disk_radiance[0]
+    np.testing.assert_allclose(np.linalg.norm(pkg.LightingState(
+        np.array([1.0, 2.0, 3.0]), 0.266, (1, 1, 1)).sun_direction_global), 1.0)
+    with pytest.raises(ValueError):
+        pkg.LightingState(np.array([0.0, 1.0, 0.0]), 6.0, (1, 1, 1))
+
+
+def test_diagnostic_view_values(pkg):
+    v = pkg.AtmosphereDiagnosticView
+    assert [int(x) for x in v] == list(range(11))
+    assert v.COMPOSITE.uses_bloom and not v.SKY_VIEW.uses_bloom
+    assert v.AERIAL_FROXEL_ONLY.is_radiance and not v.SURFACE_MASK.is_radiance
+
+
+# -------------------------------------------------------------------- C-ABI
+
+def test_library_exports_every_declared_symbol(pkg):
+    """Every function include/atmo.h declares is exported by libatmo.so and
+    bound by the ctypes layer."""
+    header = (REPO / "include" / "atmo.h").read_text()
+    declared = set(re.findall(r"^\S[^\n(]*?\b(atmo_\w+)\(", header, flags=re.M))
+    capi = __import__(pkg.__name__ + "._capi", fromlist=["_capi"])
+    assert declared == set(capi.EXPORTED_SYMBOLS)
+    lib = capi.library()
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_create_rejects_invalid_arguments_without_gpu_work(pkg):
+    capi = __import__(pkg.__name__ + "._capi", fromlist=["_capi"])
+    lib = capi.library()
+    handle = ctypes.c_void_p()
+    params = pkg.AtmosphereConfig().to_params()
+    params.sky_view_steps = 2
+    assert lib.atmo_create(ctypes.byref(params), 64, 48, ctypes.byref(handle)) == capi.ATMO_EINVAL
+    assert lib.atmo_last_error(None).decode() == "sky_view_steps must be in 4..256"
+    assert not handle.value
+    assert lib.atmo_destroy(None) == capi.ATMO_EHANDLE
+    assert lib.atmo_build_transmittance(None, 6.36e6, None) == capi.ATMO_EHANDLE
+
+
+# ------------------------------------------------------- LUT cache policy
+
+class _PolicyProbe:
+    """AtmosphereRenderer's host policy with its six kernel methods replaced
+    by recorders, so the rebuild rules run without a device."""
+
+    def __new__(cls, pkg):
+        cls_ = type("Probe", (pkg.AtmosphereRenderer,), {})
+        self = object.__new__(cls_)
+        self.calls = []
+        for name in ("_build_transmittance", "_build_multi_scattering", "_build_sky_view",
+                     "_build_aerial_perspective", "_prepare_surface_lighting", "_composite"):
+            setattr(self, name, (lambda n: (lambda *a: self.calls.append((n, a))))(name))
+        self._handle = None
+        self._planet_radius_key = None
+        self._multi_scattering_key = None
+        self._sky_key = None
+        self.luts_frozen = False
+        self.sky_snapshot_altitude_m = None
+        self.sky_snapshot_sun_cosine = None
+        self.aerial_snapshot_altitude_m = None
+        self._aerial_available = False
+        self.transmittance_rebuilds = 0
+        self.multi_scattering_rebuilds = 0
+        self.sky_view_rebuilds = 0
+        self.aerial_rebuilds = 0
+        return self
+
+
+def test_lut_cache_rebuild_policy(pkg):
+    """renderer.py:2057-2187 and the state machine exercised by
+    tests/test_m2_terrain.py:363-399."""
+    r = _PolicyProbe(pkg)
+    R, E, deg = 6_360_000.0, (4.0, 3.9, 3.7), 0.266
+    sun = np.array([0.2, 0.8, 0.4])
+    basis = (np.eye(3)[0], np.eye(3)[1], np.eye(3)[2])
+    r_cam = R + 1_000.0
+
+    def frame(rc, s=sun):
+        r.update(R, rc, s, E, deg)
+        r.prepare_frame(None, None, R, rc, s, E, deg, basis, 0.57)
+
+    frame(r_cam)
+    assert (r.transmittance_rebuilds, r.multi_scattering_rebuilds, r.sky_view_rebuilds,
+            r.aerial_rebuilds) == (1, 1, 1, 1)
+    sv_args = [a for n, a in r.calls if n == "_build_sky_view"][0]
+    assert sv_args[1] == float(np.float32(1_000.0))
+    assert sv_args[2] == float(np.float32(sun[1] / np.linalg.norm(sun)))
+    frame(r_cam)  # nothing changed: only AP (every frame) is rebuilt
+    assert (r.transmittance_rebuilds, r.multi_scattering_rebuilds, r.sky_view_rebuilds,
+            r.aerial_rebuilds) == (1, 1, 1, 2)
+    r.set_luts_frozen(True)
+    frozen = (r.sky_view_rebuilds, r.aerial_rebuilds)
+    frame(r_cam + 2.0)
+    assert (r.sky_view_rebuilds, r.aerial_rebuilds) == frozen
+    r.set_luts_frozen(False)
+    frame(r_cam + 2.0)
+    assert r.sky_view_rebuilds == frozen[0] + 1 and r.aerial_rebuilds == frozen[1] + 1
+    frame(r_cam + 4.0)
+    assert r.sky_view_rebuilds == frozen[0] + 2
+    assert r.sky_snapshot_altitude_m == float(np.float32(1_004.0))
+    frame(r_cam + 4.0, s=np.array([0.2, 0.7, 0.4]))  # sun move: SV only
+    assert r.sky_view_rebuilds == frozen[0] + 3 and r.multi_scattering_rebuilds == 1
+    r.update(R, r_cam, sun, (4.0, 3.9, 3.6), deg)  # irradiance: MS and SV
+    assert r.multi_scattering_rebuilds == 2 and r.transmittance_rebuilds == 1
+    r.update(R + 1.0, r_cam, sun, (4.0, 3.9, 3.6), deg)  # planet radius: all three
+    assert (r.transmittance_rebuilds, r.multi_scattering_rebuilds) == (2, 3)
+
+
+def test_composite_uses_sky_snapshot_altitude(pkg):
+    r = _PolicyProbe(pkg)
+    R = 6_360_000.0
+    basis = (np.eye(3)[0], np.eye(3)[1], np.eye(3)[2])
+    r.composite(None, None, None, None, None, R, R + 500.0, (0, 1, 0), (1, 1, 1), (1, 1, 1),
+                0.266, basis, 0.57, 4.0, 12.0)
+    args = r.calls[-1][1]
+    assert args[6] == 500.0 and args[7] == 500.0  # no snapshot yet: current altitude
+    r.update(R, R + 300.0, np.array([0.0, 1.0, 0.0]), (1, 1, 1), 0.266)
+    r.composite(None, None, None, None, None, R, R + 500.0, (0, 1, 0), (1, 1, 1), (1, 1, 1),
+                0.266, basis, 0.57, 4.0, 12.0)
+    args = r.calls[-1][1]
+    assert args[6] == 500.0 and args[7] == float(np.float32(300.0))
+    assert args[11] == math.radians(0.266)
+
+
+# --------------------------------------------------------- workload generator
+
+def test_frame_generator_is_seeded_and_shaped(scene):
+    a = scene.make_frame(width=320, height=180, config_index=1)
+    b = scene.make_frame(width=320, height=180, config_index=1)
+    np.testing.assert_array_equal(a.position_view, b.position_view)
+    assert a.position_view.shape == (320, 180, 3) and a.position_view.dtype == np.float32
+    assert a.surface_id.shape == (320, 180) and a.surface_id.dtype == np.int32
+    sky = a.surface_id < 0
+    assert 0.28 < sky.mean() < 0.32
+    d = np.linalg.norm(a.position_view[~sky].astype(np.float64), axis=1)
+    assert d.min() >= 9.99 and d.max() <= 500_000.0 * 1.0001
+    assert 10.0 <= a.camera_altitude_m <= 50_000.0
+    assert 0.0 <= math.degrees(math.acos(a.sun_local[1])) <= 100.0
+    # surface positions lie along the pixel rays
+    rays = scene.pixel_rays_view(320, 180, a.tangent_half_fov)
+    dirs = a.position_view[~sky] / np.linalg.norm(a.position_view[~sky], axis=1, keepdims=True)
+    np.testing.assert_allclose(dirs, rays[~sky], atol=1e-6)
+    np.testing.assert_array_equal(a.scene_hdr[sky], np.broadcast_to(
+        np.float32([0.00015, 0.0002, 0.00035]), (int(sky.sum()), 3)))
+    assert np.all(a.space_hdr[~sky] == 0.0)
+
+
+def test_view_basis_is_orthonormal(scene):
+    r, u, f = scene.view_basis_local(37.0, -12.0)
+    m = np.stack([r, u, f])
+    np.testing.assert_allclose(m @ m.T, np.eye(3), atol=1e-12)
+    np.testing.assert_allclose(np.cross(r, u), f, atol=1e-12)
+    assert f[1] == pytest.approx(math.sin(math.radians(-12.0)))

@@ -1,0 +1,75 @@
+"""CPU checks of the surrogate's reference pieces: the float64 MLP
+(oracle/mlp_fp64.py) is pinned by central finite differences and by Adam's
+first-step property; the host-side input normalisation and output transform
+round-trip."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import mlp_fp64
+
+
+def test_param_count_matches_header():
+    assert mlp_fp64.PARAM_COUNT == 9094
+
+
+def test_fp64_gradients_match_finite_differences():
+    g = np.random.default_rng(3)
+    params = mlp_fp64.init_params(3) + g.normal(0.0, 0.05, mlp_fp64.PARAM_COUNT)
+    x = g.uniform(-1, 1, (16, 5))
+    t = g.normal(0, 1, (16, 6))
+    grad, _ = mlp_fp64.gradients(params, x, t)
+    picks = g.choice(mlp_fp64.PARAM_COUNT, 60, replace=False)
+    h = 1e-6
+    for i in picks:
+        p1, p0 = params.copy(), params.copy()
+        p1[i] += h
+        p0[i] -= h
+        fd = (mlp_fp64.loss(mlp_fp64.forward(p1, x)[0], t)
+              - mlp_fp64.loss(mlp_fp64.forward(p0, x)[0], t)) / (2 * h)
+        assert fd == pytest.approx(grad[i], rel=1e-5, abs=1e-9)
+
+
+def test_silu_derivative():
+    z = np.linspace(-8, 8, 101)
+    fd = (mlp_fp64.silu(z + 1e-6) - mlp_fp64.silu(z - 1e-6)) / 2e-6
+    np.testing.assert_allclose(mlp_fp64.dsilu(z), fd, rtol=1e-7, atol=1e-10)
+
+
+def test_adam_first_step_is_signed_lr():
+    """With zero moments, Adam's first update is lr * g / (|g| + eps)."""
+    g = np.random.default_rng(0)
+    p = g.normal(size=100)
+    grad = g.normal(size=100)
+    new, m, v = mlp_fp64.adam_step(p, grad, np.zeros(100), np.zeros(100), 1, lr=1e-3)
+    np.testing.assert_allclose(p - new, 1e-3 * grad / (np.abs(grad) + 1e-8), rtol=1e-9)
+
+
+def test_input_normalisation_and_target_transform(pkg):
+    import importlib
+
+    s = importlib.import_module(pkg.__name__ + ".surrogate")
+    raw = s.sample_inputs(10_000, 1)
+    assert raw.dtype == np.float32 and raw.shape == (10_000, 5)
+    x = s.encode_host(raw)
+    assert np.all(np.abs(x) <= 1.0 + 1e-6)
+    assert x[:, 4].min() < -0.99 and x[:, 4].max() > 0.99  # log-uniform distance spans the range
+    labels = np.column_stack([np.geomspace(1e-7, 0.5, 50)] * 3 + [np.linspace(0.0, 1.0, 50)] * 3)
+    t = s.targets_from_labels(labels)
+    mean, scale = t.mean(axis=0), t.std(axis=0)
+    L, T = s.decode_outputs((t - mean) / scale, mean, scale)
+    np.testing.assert_allclose(L, labels[:, :3], rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(T, labels[:, 3:], rtol=1e-9, atol=1e-12)
+
+
+def test_init_params_layout(pkg):
+    import importlib
+
+    s = importlib.import_module(pkg.__name__ + ".surrogate")
+    p = s.init_params(0).astype(np.float64)
+    layers = mlp_fp64.unpack(p)
+    for (w, b), (fan_in, fan_out) in zip(layers, mlp_fp64.LAYERS):
+        assert w.shape == (fan_out, fan_in) and np.all(b == 0.0)
+        assert np.abs(w).max() <= math.sqrt(6.0 / fan_in)
