@@ -190,6 +190,53 @@ int32_t atmo_composite(
     float star_contrast_end, int32_t diagnostic_view, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Batched LUT builds.  The reference renders one atmosphere state per
+ * AtmosphereRenderer; a batch builds the Transmittance, Multi-scattering,
+ * Sky-view and Aerial-perspective LUTs of many independent states (optics,
+ * planet, camera, sun) at once -- exactly what, per state, a cold `update`
+ * (renderer.py:2072-2137) followed by `prepare_frame`'s aerial build
+ * (renderer.py:2173-2187) would produce.  The four stage launches are
+ * captured once into a HIP graph and replayed by atmo_batch_run.
+ * --------------------------------------------------------------------- */
+
+/* One state, in the reference's host units (float64, degrees); the library
+ * derives the float32 kernel arguments as the reference's host methods
+ * round them.  Every state of a batch must have the same LUT dimensions. */
+typedef struct AtmoBatchState {
+    AtmoParams params;
+    double planet_radius_m;
+    double camera_radius_m;
+    double sun_local[3];
+    double solar_irradiance[3];
+    double sun_angular_radius_deg;
+    double right[3];
+    double view_up[3];
+    double forward[3];
+    double tangent_half_fov;
+} AtmoBatchState;
+
+typedef struct AtmoBatch AtmoBatch;
+
+/* Validates every state (ATMO_EINVAL with the state's message, ATMO_ESHAPE
+ * for mismatched LUT sizes).  luts[0..3] are caller-owned device arrays of
+ * count copies of the T, MS, SV and AP LUTs (float32, the single-state
+ * layouts back to back); a NULL entry (or luts == NULL) makes the library
+ * allocate that array.  width / height are the frame size the
+ * aerial-perspective rays use. */
+int32_t atmo_batch_create(const AtmoBatchState* states, int32_t count, int32_t width,
+                          int32_t height, float* const luts[4], AtmoBatch** out);
+int32_t atmo_batch_destroy(AtmoBatch* batch);
+const char* atmo_batch_last_error(const AtmoBatch* batch);
+/* Builds all four LUTs of every state: one graph launch on `stream` (the
+ * graph is captured on the first call). */
+int32_t atmo_batch_run(AtmoBatch* batch, void* stream);
+/* Device address and shape of the batch's LUT array `which`
+ * (ATMO_BUF_TRANSMITTANCE .. ATMO_BUF_AERIAL_SCATTERING): shape[0] = count,
+ * then the single-state LUT shape. */
+int32_t atmo_batch_get_buffer(AtmoBatch* batch, int32_t which, void** device_ptr,
+                              int64_t shape[5]);
+
+/* ------------------------------------------------------------------------
  * Learned surrogate of the composite's surface branch.  New in this library:
  * the reference has no learned path.  The surrogate replaces, per surface
  * pixel, the direct limb integral (_integrate_surface_scattering,
@@ -204,19 +251,24 @@ int32_t atmo_composite(
 
 /* Input normalisation, training domain and output decoding.  Inputs are
  * (camera altitude h, view-zenith cosine, sun-zenith cosine, relative
- * azimuth phi, segment length d) mapped to [-1, 1] (h / altitude_max_m,
- * phi / pi and log d over [distance_min_m, distance_max_m] linearly).
- * Outputs y (6) decode as L_c = exp(y_c * out_scale_c + out_mean_c) -
- * eps_inscatter (in-scatter per unit irradiance, c = 0..2) and
- * T_c = exp(y_{3+c} * out_scale + out_mean) - eps_transmittance. */
+ * azimuth phi, segment length d) mapped to [-1, 1]: altitude as
+ * log(1 + h / altitude_scale_m) over [0, altitude_max_m], phi / pi, and
+ * log d over [distance_min_m, distance_max_m], each linearly.
+ * Outputs y (6) are standardised: v = y * out_scale + out_mean.  With
+ * output_mode 1 (linear) L_c = max(v_c, 0) (in-scatter per unit
+ * irradiance, c = 0..2) and T_c = clamp(v_{3+c}, 0, 1); with output_mode 0
+ * (log) L_c = exp(v_c) - eps_inscatter and T_c = exp(v_{3+c}) -
+ * eps_transmittance. */
 typedef struct AtmoSurrogateTransform {
     float altitude_max_m;
+    float altitude_scale_m;
     float distance_min_m;
     float distance_max_m;
     float out_mean[6];
     float out_scale[6];
     float eps_inscatter;
     float eps_transmittance;
+    int32_t output_mode;
 } AtmoSurrogateTransform;
 
 typedef struct AtmoMlp AtmoMlp;
@@ -236,16 +288,19 @@ int32_t atmo_mlp_set_transform(AtmoMlp* mlp, const AtmoSurrogateTransform* trans
 int32_t atmo_mlp_encode(AtmoMlp* mlp, const float* raw, int64_t n, float* x, void* stream);
 /* FP32 training forward pass: x (B, 5) encoded -> y (B, 6) raw outputs. */
 int32_t atmo_mlp_forward(AtmoMlp* mlp, const float* x, int32_t batch, float* y, void* stream);
-/* Gradient of the mean-square error over rows index[0..B) (index may be
- * NULL for rows 0..B) of x (.., 5) and target (.., 6); grad_out (device,
- * ATMO_MLP_PARAMS) and loss_out (device, 1 float) may be NULL. */
+/* Gradient of the weighted mean-square error
+ * sum(weight * (y - target)^2) / (B * 6) over rows index[0..B) (index may
+ * be NULL for rows 0..B) of x (.., 5), target (.., 6) and weight (.., 6;
+ * NULL = all ones); grad_out (device, ATMO_MLP_PARAMS) and loss_out (device,
+ * 1 float) may be NULL. */
 int32_t atmo_mlp_gradients(AtmoMlp* mlp, const float* x, const float* target,
-                           const int32_t* index, int32_t batch, float* grad_out,
-                           float* loss_out, void* stream);
-/* One Adam step (lr, betas 0.9 / 0.999, eps 1e-8; step counts from 1). */
+                           const float* weight, const int32_t* index, int32_t batch,
+                           float* grad_out, float* loss_out, void* stream);
+/* One Adam step (lr, betas 0.9 / 0.999, eps 1e-8; step counts from 1) on
+ * the weighted mean-square error of atmo_mlp_gradients. */
 int32_t atmo_mlp_train_step(AtmoMlp* mlp, const float* x, const float* target,
-                            const int32_t* index, int32_t batch, float lr, int32_t step,
-                            float* loss_out, void* stream);
+                            const float* weight, const int32_t* index, int32_t batch,
+                            float lr, int32_t step, float* loss_out, void* stream);
 /* Repacks the current parameters into BF16 MFMA fragments for inference. */
 int32_t atmo_mlp_pack(AtmoMlp* mlp, void* stream);
 /* BF16 MFMA inference (FP32 accumulation) on encoded inputs x (n, 5). */

@@ -40,6 +40,11 @@ EXPORTED_SYMBOLS = (
     "atmo_build_aerial_perspective",
     "atmo_prepare_surface_lighting",
     "atmo_composite",
+    "atmo_batch_create",
+    "atmo_batch_destroy",
+    "atmo_batch_last_error",
+    "atmo_batch_run",
+    "atmo_batch_get_buffer",
     "atmo_mlp_create",
     "atmo_mlp_destroy",
     "atmo_mlp_last_error",
@@ -64,12 +69,14 @@ class AtmoSurrogateTransform(ctypes.Structure):
 
     _fields_ = [
         ("altitude_max_m", ctypes.c_float),
+        ("altitude_scale_m", ctypes.c_float),
         ("distance_min_m", ctypes.c_float),
         ("distance_max_m", ctypes.c_float),
         ("out_mean", ctypes.c_float * 6),
         ("out_scale", ctypes.c_float * 6),
         ("eps_inscatter", ctypes.c_float),
         ("eps_transmittance", ctypes.c_float),
+        ("output_mode", ctypes.c_int32),
     ]
 
 
@@ -107,6 +114,24 @@ class AtmoParams(ctypes.Structure):
     ]
 
 
+class AtmoBatchState(ctypes.Structure):
+    """struct AtmoBatchState (include/atmo.h): one state in host units
+    (float64, degrees)."""
+
+    _fields_ = [
+        ("params", AtmoParams),
+        ("planet_radius_m", ctypes.c_double),
+        ("camera_radius_m", ctypes.c_double),
+        ("sun_local", ctypes.c_double * 3),
+        ("solar_irradiance", ctypes.c_double * 3),
+        ("sun_angular_radius_deg", ctypes.c_double),
+        ("right", ctypes.c_double * 3),
+        ("view_up", ctypes.c_double * 3),
+        ("forward", ctypes.c_double * 3),
+        ("tangent_half_fov", ctypes.c_double),
+    ]
+
+
 F3 = ctypes.c_float * 3
 _P = ctypes.c_void_p
 _F = ctypes.c_float
@@ -136,6 +161,13 @@ def _signatures(lib: ctypes.CDLL) -> None:
         _F, _F, _F, _I, _P,
     ]
     _PI = ctypes.POINTER(ctypes.c_int32)
+    lib.atmo_batch_create.argtypes = [ctypes.POINTER(AtmoBatchState), _I, _I, _I,
+                                      ctypes.POINTER(_P), ctypes.POINTER(_P)]
+    lib.atmo_batch_destroy.argtypes = [_P]
+    lib.atmo_batch_last_error.argtypes = [_P]
+    lib.atmo_batch_last_error.restype = ctypes.c_char_p
+    lib.atmo_batch_run.argtypes = [_P, _P]
+    lib.atmo_batch_get_buffer.argtypes = [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int64)]
     lib.atmo_mlp_create.argtypes = [_I, ctypes.POINTER(_P)]
     lib.atmo_mlp_destroy.argtypes = [_P]
     lib.atmo_mlp_last_error.argtypes = [_P]
@@ -145,8 +177,8 @@ def _signatures(lib: ctypes.CDLL) -> None:
     lib.atmo_mlp_set_transform.argtypes = [_P, ctypes.POINTER(AtmoSurrogateTransform)]
     lib.atmo_mlp_encode.argtypes = [_P, _P, ctypes.c_int64, _P, _P]
     lib.atmo_mlp_forward.argtypes = [_P, _P, _I, _P, _P]
-    lib.atmo_mlp_gradients.argtypes = [_P, _P, _P, _P, _I, _P, _P, _P]
-    lib.atmo_mlp_train_step.argtypes = [_P, _P, _P, _P, _I, _F, _I, _P, _P]
+    lib.atmo_mlp_gradients.argtypes = [_P, _P, _P, _P, _P, _I, _P, _P, _P]
+    lib.atmo_mlp_train_step.argtypes = [_P, _P, _P, _P, _P, _I, _F, _I, _P, _P]
     lib.atmo_mlp_pack.argtypes = [_P, _P]
     lib.atmo_mlp_infer.argtypes = [_P, _P, ctypes.c_int64, _P, _P]
     lib.atmo_surrogate_labels.argtypes = [_P, _P, ctypes.c_int64, _F, _F, _P, _P]
@@ -155,7 +187,7 @@ def _signatures(lib: ctypes.CDLL) -> None:
         _F, _F, _F, _P,
     ]
     for name in EXPORTED_SYMBOLS:
-        if name not in ("atmo_last_error", "atmo_mlp_last_error"):
+        if name not in ("atmo_last_error", "atmo_mlp_last_error", "atmo_batch_last_error"):
             getattr(lib, name).restype = ctypes.c_int32
 
 
@@ -198,6 +230,7 @@ def check(status: int, handle=None) -> None:
 
 
 __all__ = [
+    "AtmoBatchState",
     "AtmoParams",
     "AtmoError",
     "EXPORTED_SYMBOLS",

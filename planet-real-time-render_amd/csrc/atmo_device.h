@@ -9,12 +9,12 @@
 // device exp/acos/cos/sin/pow/atan2.
 //
 // Elementary functions are correctly rounded float32 (cr_*: evaluated in
-// float64, rounded once), the same definition the CPU restatement uses.
-// The reference's formulas amplify last-bit differences of exp / acos / cos
-// by up to ~1e3 (finite-disk visibility near the horizon, (1 - exp(-x)) / s
-// for small x, the sky-view horizon coordinate), so a float32 libm whose
-// results differ in the last bit from another's cannot meet a 1e-4 parity
-// bar there; correctly rounded results are unique.
+// float64, rounded once), the definition the CPU restatement uses, wherever
+// the reference's formulas amplify last-bit differences: finite-disk
+// visibility and the representative sun zenith near the horizon,
+// (1 - exp(-x)) / s for small x, the sky-view horizon coordinate.  Where
+// nothing downstream amplifies a last-bit difference (density exp, exp of
+// long segments) the hardware-rate function is used; each such site says so.
 //
 // Frame- and LUT-sized arrays use the reference's Taichi field layout:
 // (X, Y[, Z], 3) float32, element [x, y] at (x * Y + y) * 3.
@@ -91,8 +91,10 @@ __device__ __forceinline__ void st3(float* __restrict__ base, long i, f3 v) {
 __device__ __forceinline__ f3 density(const Consts& k, float h) {
     float rayleigh = 0.0f, mie = 0.0f, absorption = 0.0f;
     if (0.0f <= h && h <= k.top) {
-        rayleigh = cr_exp(-h / k.h_r);
-        mie = cr_exp(-h / k.h_m);
+        // Hardware-rate exp: a last-bit difference here moves the
+        // extinction by one ulp, which no later step amplifies.
+        rayleigh = expf(-h / k.h_r);
+        mie = expf(-h / k.h_m);
         absorption = fmaxf(1.0f - fabsf(h - k.peak) / k.half_width, 0.0f);
     }
     return f3{rayleigh, mie, absorption};
@@ -446,11 +448,22 @@ __device__ __forceinline__ SunGeom sun_geometry(const Consts& k, float h, float 
     SunGeom g;
     float radius = bottom + fmaxf(h, 0.0f);
     float hc = -shell_rho(h, bottom) / fmaxf(radius, 1.0f);
+    // Both zeniths are correctly rounded: near the horizon the representative
+    // zenith feeds cos() and the T-LUT distance coordinate, whose slope there
+    // is ~planet radius per unit cosine.
     g.horizon_zenith = cr_acos(clampf(hc, -1.0f, 1.0f));
     g.sun_zenith = cr_acos(clampf(sun_cosine, -1.0f, 1.0f));
     float sep = g.horizon_zenith - g.sun_zenith;
     float n = clampf(sep / fmaxf(angular_radius, 1.0e-6f), -1.0f, 1.0f);
-    g.visibility = (cr_acos(-n) + n * sqrtf(fmaxf(1.0f - n * n, 0.0f))) / k.pi;
+    // Outside the penumbra n is exactly +-1 and the visible fraction exactly
+    // 1 or 0 (acos(-1) / pi and acos(1) / pi); only inside it does acos(-n)
+    // need evaluating, where its terms cancel.
+    if (n >= 1.0f)
+        g.visibility = 1.0f;
+    else if (n <= -1.0f)
+        g.visibility = 0.0f;
+    else
+        g.visibility = (cr_acos(-n) + n * sqrtf(fmaxf(1.0f - n * n, 0.0f))) / k.pi;
     return g;
 }
 
@@ -474,9 +487,33 @@ __device__ __forceinline__ float mie_phase(const Consts& k, float c) {
     return k.mie_c * (1.0f + c * c) / (k.mie_d * cr_pow(den, 1.5f));
 }
 
+// exp(-x) for the segment transmittance, x in [0, 80].  For x < 1/16 the
+// reference's (1 - exp(-x)) / sigma cancels, so the last bit of exp(-x) is
+// amplified by 1/x: there exp(-x) is its Taylor series evaluated in float64
+// (truncation < 4e-17) and rounded once, i.e. correctly rounded.  Elsewhere
+// the hardware-rate expf is within an ulp, amplified at most 16-fold.
+__device__ __forceinline__ float exp_neg_segment(float x) {
+    if (x < 0.0625f) {
+        const double t = -(double)x;
+        double p = 1.0 / 362880.0;
+        p = fma(p, t, 1.0 / 40320.0);
+        p = fma(p, t, 1.0 / 5040.0);
+        p = fma(p, t, 1.0 / 720.0);
+        p = fma(p, t, 1.0 / 120.0);
+        p = fma(p, t, 1.0 / 24.0);
+        p = fma(p, t, 1.0 / 6.0);
+        p = fma(p, t, 0.5);
+        p = fma(p, t, 1.0);
+        p = fma(p, t, 1.0);
+        return (float)p;
+    }
+    return expf(-x);
+}
+
 // renderer.py:806-816
 __device__ __forceinline__ void segment_integral(f3 ext, float step, f3& trans, f3& integ) {
-    trans = exp3(-mins(ext * step, 80.0f));
+    f3 x = mins(ext * step, 80.0f);
+    trans = f3{exp_neg_segment(x.x), exp_neg_segment(x.y), exp_neg_segment(x.z)};
     integ.x = ext.x > 1.0e-12f ? (1.0f - trans.x) / ext.x : step;
     integ.y = ext.y > 1.0e-12f ? (1.0f - trans.y) / ext.y : step;
     integ.z = ext.z > 1.0e-12f ? (1.0f - trans.z) / ext.z : step;
@@ -715,6 +752,59 @@ __device__ __forceinline__ f3 sample_aerial(const Consts& k, const float* __rest
     f3 low = a * (1.0f - fy) + b * fy;
     f3 high = c * (1.0f - fy) + e * fy;
     return low * (1.0f - fz) + high * fz;
+}
+
+// Sky branch of the COMPOSITE view (renderer.py:1882-1940): sky-view
+// radiance, scene and stars through the camera transmittance with the
+// star-contrast fade, plus the finite sun disk with a one-pixel filter.
+__device__ __forceinline__ f3 composite_sky(const Consts& k, const float* __restrict__ tlut,
+                                            const float* __restrict__ svlut, f3 ray, f3 color,
+                                            f3 space, float cam_alt, float sky_alt, f3 sun, f3 disk,
+                                            float ar, float thf, int height, float c0, float c1,
+                                            float bottom) {
+    f3 trans = camera_transmittance(k, tlut, cam_alt, ray, bottom);
+    f3 sky = sample_sky_view(k, svlut, ray, sun, sky_alt, bottom);
+    f3 star = space * trans;
+    const f3 lum = mk(0.2126f, 0.7152f, 0.0722f);
+    float contrast = dot(star, lum) / fmaxf(dot(sky, lum), 1.0e-5f);
+    float vis = clampf((contrast - c0) / fmaxf(c1 - c0, 1.0e-5f), 0.0f, 1.0f);
+    vis = vis * vis * (3.0f - 2.0f * vis);
+    color = (sky + color * trans) + star * vis;
+    float angular_distance = cr_acos(clampf(dot(ray, sun), -1.0f, 1.0f));
+    float pixel_filter = cr_atan2(thf * 1.41421356f / (float)height, 1.0f);
+    float cov = clampf((ar + pixel_filter - angular_distance) / fmaxf(2.0f * pixel_filter, 1.0e-8f),
+                       0.0f, 1.0f);
+    cov = cov * cov * (3.0f - 2.0f * cov);
+    return color + (disk * trans) * cov;
+}
+
+// Pixel-centre ray (renderer.py:1850-1859), y = 0 the bottom row.
+__device__ __forceinline__ f3 camera_ray(int px, int py, int width, int height, f3 right, f3 up,
+                                         f3 forward, float thf) {
+    float aspect = (float)width / (float)height;
+    float su = ((float)px + 0.5f) / (float)width;
+    float sv = ((float)py + 0.5f) / (float)height;
+    float sx = (su * 2.0f - 1.0f) * aspect;
+    float sy = sv * 2.0f - 1.0f;
+    return normalized((right * sx + up * sy) + forward / thf);
+}
+
+// Compacts the flagged threads of a 256-thread block into list[0..count) in
+// thread order (wave ballots and per-wave offsets); returns count.  Every
+// thread of the block must call it.
+__device__ __forceinline__ int compact_block(bool flag, int* list, int* wave_counts) {
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const unsigned long long mask = __ballot(flag);
+    if (lane == 0) wave_counts[wave] = __popcll(mask);
+    __syncthreads();
+    int offset = 0, total = 0;
+    for (int w = 0; w < 4; ++w) {
+        offset += w < wave ? wave_counts[w] : 0;
+        total += wave_counts[w];
+    }
+    if (flag) list[offset + __popcll(mask & ((1ull << lane) - 1ull))] = threadIdx.x;
+    __syncthreads();
+    return total;
 }
 
 }  // namespace atmo

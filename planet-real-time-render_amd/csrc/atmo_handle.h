@@ -25,8 +25,49 @@ struct AtmoHandle {
     float* sun_t;
     float* sky_l;
     bool borrowed[6];  // buffer supplied by the caller through atmo_bind_buffer
+    int* queue;        // surrogate composite: [0] = count, then out-of-domain pixel indices
     char error[512];
 };
+
+// Arguments of the composite kernels (renderer.py:1827-1849).
+struct CompositeArgs {
+    const float* scene;
+    const float* space;
+    const int32_t* sid;
+    const float* pos;
+    float* hdr;
+    const float* tlut;
+    const float* mslut;
+    const float* svlut;
+    const float* aplut;
+    int width, height;
+    float bottom, cam_alt, sky_alt;
+    atmo::f3 sun, irradiance, disk;
+    float ar;
+    atmo::f3 right, up, forward;
+    float thf, c0, c1;
+    int view;
+};
+
+// Float32 kernel arguments of one state's LUT builds, derived on the host
+// exactly as the reference's update / prepare_frame round them
+// (renderer.py:2082-2187).
+struct BatchState {
+    atmo::Consts k;
+    float bottom;          // planet radius
+    atmo::f3 irradiance;   // solar irradiance
+    float ar_update;       // float32(radians(alpha)), update()'s MS / SV argument
+    float sky_alt;         // float32(max(r - R, 0)): sky-view altitude key
+    float sun_zc;          // float32(clip(normalised sun.y)): sky-view sun key
+    float cam_alt;         // max(r - R, 0): aerial-perspective altitude
+    atmo::f3 sun;          // sun_local as given (aerial perspective)
+    float ar;              // radians(alpha) (aerial perspective)
+    atmo::f3 right, up, forward;
+    float thf;
+};
+
+// Launches the sky pass of the COMPOSITE view (atmo_kernels.hip).
+int32_t atmo_launch_composite_sky(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream);
 
 inline atmo::f3 mk3(const float* a) { return atmo::f3{a[0], a[1], a[2]}; }
 

@@ -54,6 +54,7 @@ struct AtmoMlp {
     float* a[3];     // hidden activations silu(z) [B, 64]
     float* dz[4];    // d loss / d z per layer [B, out]
     float* xbuf;     // gathered inputs [B, 5]
+    float* wbuf;     // gathered loss weights [B, 6]
     float* partial;  // weight-gradient partial sums [B / kChunk, 64 * 64 + 64]
     float* loss;     // [B / 256 + 1] block partials, loss at [0] after reduce
     uint16_t* frag;  // bf16 MFMA weight fragments for inference
@@ -88,7 +89,9 @@ __device__ __forceinline__ void sample_geometry(float mu, float mu_s, float phi,
 
 __device__ __forceinline__ void encode(const AtmoSurrogateTransform& t, float h, float mu,
                                        float mu_s, float phi, float d, float x[5]) {
-    x[0] = h * (2.0f / t.altitude_max_m) - 1.0f;
+    x[0] = log1pf(fmaxf(h, 0.0f) / t.altitude_scale_m) *
+               (2.0f / log1pf(t.altitude_max_m / t.altitude_scale_m)) -
+           1.0f;
     x[1] = mu;
     x[2] = mu_s;
     x[3] = phi * (2.0f / 3.14159265358979f) - 1.0f;
@@ -169,19 +172,22 @@ __global__ void __launch_bounds__(256) k_dense_forward(int B, int K, int N, cons
     if (a) a[e] = silu(acc);
 }
 
-// Mean-square-error gradient dz = 2 (y - t) / (B * 6) and per-block loss sums.
-// `t` and `dz` may alias (the target is overwritten by its gradient).
+// Weighted mean-square-error gradient dz = 2 w (y - t) / (B * 6) and
+// per-block sums of w (y - t)^2.  `t` and `dz` may alias (the target is
+// overwritten by its gradient); `w` NULL means unit weights.
 __global__ void __launch_bounds__(256) k_mse(int B, const float* __restrict__ y, const float* t,
-                                             float* dz, float* __restrict__ block_loss) {
+                                             const float* __restrict__ w, float* dz,
+                                             float* __restrict__ block_loss) {
     __shared__ float red[256];
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)B * 6;
-    float d = 0.0f;
+    float d = 0.0f, we = 0.0f;
     if (e < n) {
         d = y[e] - t[e];
-        dz[e] = 2.0f * d / (float)n;
+        we = w ? w[e] : 1.0f;
+        dz[e] = 2.0f * we * d / (float)n;
     }
-    red[threadIdx.x] = d * d;
+    red[threadIdx.x] = we * d * d;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
@@ -287,6 +293,14 @@ __device__ __forceinline__ short to_bf16(float f) {
     return (short)(u >> 16);
 }
 
+typedef __bf16 bf16v4 __attribute__((ext_vector_type(4)));
+
+// Four floats to packed bf16, round to nearest even: two v_cvt_pk_bf16_f32
+// on gfx950.
+__device__ __forceinline__ bf16x4 pack_bf16(floatx4 v) {
+    return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bf16v4));
+}
+
 __host__ __device__ constexpr int frag_index(int layer, int mt, int kt) {
     // layer 0: 4 x 1, layers 1-2: 4 x 4, layer 3: 1 x 4 fragments
     return layer == 0 ? mt : layer == 1 ? 4 + mt * 4 + kt : layer == 2 ? 20 + mt * 4 + kt : 36 + kt;
@@ -326,23 +340,28 @@ __device__ __forceinline__ floatx4 mfma(bf16x4 a, bf16x4 b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// SiLU z / (1 + e^-z) at hardware rate (v_exp_f32, v_rcp_f32), then
+// packed to bf16 as the next layer's B operand.
 __device__ __forceinline__ bf16x4 silu_bf16(floatx4 c) {
-    bf16x4 r;
+    floatx4 r;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         float z = c[i];
-        float e = __builtin_amdgcn_exp2f(-1.44269504f * z);
-        r[i] = to_bf16(z * __builtin_amdgcn_rcpf(1.0f + e));
+        r[i] = z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504f * z));
     }
-    return r;
+    return pack_bf16(r);
 }
 
 // Runs the MLP for the 64 pixels of a wave.  `x` holds the lane's own
-// pixel features; returns the 6 raw outputs of the lane's pixel.  The
-// weights are the A operand and activations the B operand, so the output
+// pixel features; returns the 6 raw outputs of the lane's pixel.  Layer 1
+// (5 -> 64) runs in FP32 on the VALU: bf16 inputs would quantise log(d) by
+// ~0.4 %, i.e. the segment length by ~3 %.  Layers 2-4 run on MFMA with the
+// weights as the A operand and activations as the B operand, so the output
 // tile of one layer (lane l: neurons 4 (l / 16) + i of pixel l % 16) is
 // exactly the B fragment of the next layer's K-step: no data movement.
+// w1 holds W1 row-major (64 x 5), bias the fp32 biases of all layers.
 __device__ __forceinline__ void mlp_wave(const float x[5], const bf16x4* __restrict__ frag,
+                                         const float* __restrict__ w1,
                                          const float* __restrict__ bias, float out[6]) {
     const int lane = threadIdx.x % 64;
     const int group = lane / 16, col = lane % 16;
@@ -352,18 +371,18 @@ __device__ __forceinline__ void mlp_wave(const float x[5], const bf16x4* __restr
         float f[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) f[q] = __shfl(x[q], src, 64);
-        bf16x4 in;
-        in[0] = group == 0 ? to_bf16(f[0]) : group == 1 ? to_bf16(f[4]) : (short)0;
-        in[1] = group == 0 ? to_bf16(f[1]) : (short)0;
-        in[2] = group == 0 ? to_bf16(f[2]) : (short)0;
-        in[3] = group == 0 ? to_bf16(f[3]) : (short)0;
         bf16x4 act[4];
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
             floatx4 c;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) c[i] = bias[16 * mt + 4 * group + i];
-            c = mfma(frag[frag_index(0, mt, 0) * 64 + lane], in, c);
+            for (int i = 0; i < 4; ++i) {
+                const int n = 16 * mt + 4 * group + i;
+                float acc = bias[n];
+#pragma unroll
+                for (int q = 0; q < 5; ++q) acc += f[q] * w1[n * 5 + q];
+                c[i] = acc;
+            }
             act[mt] = silu_bf16(c);
         }
 #pragma unroll
@@ -406,6 +425,11 @@ __device__ __forceinline__ void decode(const AtmoSurrogateTransform& t, const fl
     float y[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) y[q] = raw[q] * t.out_scale[q] + t.out_mean[q];
+    if (t.output_mode == 1) {
+        L = maxs(mk(y[0], y[1], y[2]), 0.0f);
+        T = clamp3(mk(y[3], y[4], y[5]), 0.0f, 1.0f);
+        return;
+    }
     L = mk(fmaxf(expf(y[0]) - t.eps_inscatter, 0.0f), fmaxf(expf(y[1]) - t.eps_inscatter, 0.0f),
            fmaxf(expf(y[2]) - t.eps_inscatter, 0.0f));
     T = clamp3(mk(expf(y[3]) - t.eps_transmittance, expf(y[4]) - t.eps_transmittance,
@@ -429,103 +453,127 @@ struct SurrogateArgs {
     f3 right, up, forward;
     float thf, c0, c1;
     const uint16_t* frag;
+    const float* w1;
     const float* bias;
     AtmoSurrogateTransform t;
 };
 
 // The COMPOSITE view with the surrogate in place of the surface branch's
-// in-scatter and transmittance.  One thread per pixel, waves of 64
-// consecutive pixels (the block is 256 threads; every lane reaches the
-// MFMA calls, which need the whole wave).
-__global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, SurrogateArgs a) {
+// in-scatter and transmittance.  Persistent blocks: the 20 KiB of weight
+// fragments are staged in LDS once per block, which then walks 256-pixel
+// groups.  In a group, sky pixels are finished by their own thread (the
+// plain sky branch); surface pixels are compacted onto the leading threads,
+// so the network runs in full waves over surface pixels only (the MFMA chain
+// needs the whole wave; a wave runs it when any lane holds an in-domain
+// pixel).  Out-of-domain surface pixels are queued for
+// k_composite_fallback, which keeps the plain integrator's registers out of
+// this kernel.
+__global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, SurrogateArgs a,
+                                                             int* __restrict__ queue) {
     __shared__ bf16x4 frag_s[kFragments * 64];
     __shared__ float bias_s[64 * 3 + 16];
+    __shared__ float w1_s[64 * 5];
+    __shared__ int list[256];
+    __shared__ int wave_counts[4];
     const bf16x4* fg = reinterpret_cast<const bf16x4*>(a.frag);
     for (int e = threadIdx.x; e < kFragments * 64; e += blockDim.x) frag_s[e] = fg[e];
     for (int e = threadIdx.x; e < 64 * 3 + 16; e += blockDim.x) bias_s[e] = a.bias[e];
-    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 5; e += blockDim.x) w1_s[e] = a.w1[e];
+    // (the first compaction's barriers order this staging before any read)
 
     const long npix = (long)a.width * a.height;
-    const long i_raw = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i_raw < npix;
-    const long i = live ? i_raw : npix - 1;
-    const int px = (int)(i / a.height), py = (int)(i % a.height);
-    const float aspect = (float)a.width / (float)a.height;
-    const float su = ((float)px + 0.5f) / (float)a.width;
-    const float sv = ((float)py + 0.5f) / (float)a.height;
-    const float sx = (su * 2.0f - 1.0f) * aspect;
-    const float sy = sv * 2.0f - 1.0f;
-    const f3 ray = normalized((a.right * sx + a.up * sy) + a.forward / a.thf);
-    const bool surface = a.sid[i] >= 0;
-    const float sd = surface ? norm(ld3(a.pos, i)) : 1.0f;
-    const bool learned = live && surface && in_domain(a.t, a.cam_alt, sd);
-
-    // features of this lane's pixel (relative azimuth as in the sky-view
-    // sampler, renderer.py:1482-1497)
-    float x[5];
-    {
-        float vl = sqrtf(ray.x * ray.x + ray.z * ray.z);
-        float sl = sqrtf(a.sun.x * a.sun.x + a.sun.z * a.sun.z);
-        float phi = 0.0f;
-        if (vl > 1.0e-6f && sl > 1.0e-6f)
-            phi = acosf(clampf((ray.x * a.sun.x + ray.z * a.sun.z) / (vl * sl), -1.0f, 1.0f));
-        encode(a.t, a.cam_alt, clampf(ray.y, -1.0f, 1.0f), clampf(a.sun.y, -1.0f, 1.0f), phi,
-               fmaxf(sd, a.t.distance_min_m), x);
-    }
-    float raw[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    // the wave runs the network if any of its pixels needs it
-    if (__ballot(learned)) mlp_wave(x, frag_s, bias_s, raw);
-    if (!live) return;
-
-    f3 color = ld3(a.scene, i);
-    if (surface) {
-        f3 L, T;
-        if (learned) {
-            decode(a.t, raw, L, T);
-            L = L * a.irradiance;
-        } else {  // outside the training domain: the plain integrator
-            L = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun, a.irradiance,
-                                  a.bottom, a.ar);
-            T = surface_transmittance(k, a.tlut, a.cam_alt, ray, sd, a.bottom);
+    const long groups = (npix + 255) / 256;
+    const int lane = threadIdx.x % 64;
+    for (long g = blockIdx.x; g < groups; g += gridDim.x) {
+        const long base = g * 256;
+        const long i = base + threadIdx.x;
+        const bool live = i < npix;
+        const bool surface = live && a.sid[i] >= 0;
+        if (live && !surface) {
+            f3 ray = camera_ray((int)(i / a.height), (int)(i % a.height), a.width, a.height,
+                                a.right, a.up, a.forward, a.thf);
+            f3 c = composite_sky(k, a.tlut, a.svlut, ray, ld3(a.scene, i), ld3(a.space, i),
+                                 a.cam_alt, a.sky_alt, a.sun, a.disk, a.ar, a.thf, a.height, a.c0,
+                                 a.c1, a.bottom);
+            st3(a.hdr, i, maxs(c, 0.0f));
         }
-        color = L + color * T;
-    } else {
-        f3 trans = camera_transmittance(k, a.tlut, a.cam_alt, ray, a.bottom);
-        f3 sky = sample_sky_view(k, a.svlut, ray, a.sun, a.sky_alt, a.bottom);
-        f3 star = ld3(a.space, i) * trans;
-        const f3 lum = mk(0.2126f, 0.7152f, 0.0722f);
-        float contrast = dot(star, lum) / fmaxf(dot(sky, lum), 1.0e-5f);
-        float vis = clampf((contrast - a.c0) / fmaxf(a.c1 - a.c0, 1.0e-5f), 0.0f, 1.0f);
-        vis = vis * vis * (3.0f - 2.0f * vis);
-        color = (sky + color * trans) + star * vis;
-        float angular_distance = cr_acos(clampf(dot(ray, a.sun), -1.0f, 1.0f));
-        float pixel_filter = cr_atan2(a.thf * 1.41421356f / (float)a.height, 1.0f);
-        float cov = clampf((a.ar + pixel_filter - angular_distance) / fmaxf(2.0f * pixel_filter, 1.0e-8f),
-                           0.0f, 1.0f);
-        cov = cov * cov * (3.0f - 2.0f * cov);
-        color = color + (a.disk * trans) * cov;
+        const int count = compact_block(surface, list, wave_counts);
+        const bool has = (int)threadIdx.x < count;
+        const long j = base + (has ? list[threadIdx.x] : 0);
+        const f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height,
+                                  a.right, a.up, a.forward, a.thf);
+        const float sd = has ? norm(ld3(a.pos, j)) : 1.0f;
+        const bool learned = has && in_domain(a.t, a.cam_alt, sd);
+        // queue the out-of-domain surface pixels (one atomic per wave)
+        const unsigned long long fb = __ballot(has && !learned);
+        if (fb) {
+            int slot = 0;
+            if (lane == 0) slot = atomicAdd(queue, __popcll(fb));
+            slot = __shfl(slot, 0, 64);
+            if (has && !learned) queue[1 + slot + __popcll(fb & ((1ull << lane) - 1ull))] = (int)j;
+        }
+        if (__ballot(learned)) {  // wave-uniform branch
+            float x[5];
+            // relative azimuth as in the sky-view sampler (renderer.py:1482-1497)
+            float vl = sqrtf(ray.x * ray.x + ray.z * ray.z);
+            float sl = sqrtf(a.sun.x * a.sun.x + a.sun.z * a.sun.z);
+            float phi = 0.0f;
+            if (vl > 1.0e-6f && sl > 1.0e-6f)
+                phi = acosf(clampf((ray.x * a.sun.x + ray.z * a.sun.z) / (vl * sl), -1.0f, 1.0f));
+            encode(a.t, a.cam_alt, clampf(ray.y, -1.0f, 1.0f), clampf(a.sun.y, -1.0f, 1.0f), phi,
+                   fmaxf(sd, a.t.distance_min_m), x);
+            float raw[6];
+            mlp_wave(x, frag_s, w1_s, bias_s, raw);
+            if (learned) {
+                f3 L, T;
+                decode(a.t, raw, L, T);
+                st3(a.hdr, j, maxs(L * a.irradiance + ld3(a.scene, j) * T, 0.0f));
+            }
+        }
+        __syncthreads();  // `list` is rewritten by the next group
     }
-    st3(a.hdr, i, maxs(color, 0.0f));
+}
+
+__global__ void k_zero_queue(int* __restrict__ queue) { queue[0] = 0; }
+
+// Out-of-domain surface pixels of the surrogate composite: the plain
+// surface branch (renderer.py:1863-1881).
+__global__ void __launch_bounds__(256) k_composite_fallback(Consts k, SurrogateArgs a,
+                                                            const int* __restrict__ queue) {
+    const int count = queue[0];
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < count; e += gridDim.x * blockDim.x) {
+        const long j = queue[1 + e];
+        const f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height,
+                                  a.right, a.up, a.forward, a.thf);
+        const float sd = norm(ld3(a.pos, j));
+        f3 L = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun, a.irradiance,
+                                 a.bottom, a.ar);
+        f3 T = surface_transmittance(k, a.tlut, a.cam_alt, ray, sd, a.bottom);
+        st3(a.hdr, j, maxs(L + ld3(a.scene, j) * T, 0.0f));
+    }
 }
 
 // Batch inference on encoded features (testing and evaluation): x [n, 5]
 // -> y [n, 6] raw network outputs, through the same MFMA path.
 __global__ void __launch_bounds__(256) k_mlp_infer(long n, const float* __restrict__ x,
                                                    const uint16_t* __restrict__ frag,
+                                                   const float* __restrict__ w1,
                                                    const float* __restrict__ bias,
                                                    float* __restrict__ y) {
     __shared__ bf16x4 frag_s[kFragments * 64];
     __shared__ float bias_s[64 * 3 + 16];
+    __shared__ float w1_s[64 * 5];
     const bf16x4* fg = reinterpret_cast<const bf16x4*>(frag);
     for (int e = threadIdx.x; e < kFragments * 64; e += blockDim.x) frag_s[e] = fg[e];
     for (int e = threadIdx.x; e < 64 * 3 + 16; e += blockDim.x) bias_s[e] = bias[e];
+    for (int e = threadIdx.x; e < 64 * 5; e += blockDim.x) w1_s[e] = w1[e];
     __syncthreads();
     const long i_raw = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long i = i_raw < n ? i_raw : n - 1;
     float f[5];
     for (int q = 0; q < 5; ++q) f[q] = x[i * 5 + q];
     float out[6];
-    mlp_wave(f, frag_s, bias_s, out);
+    mlp_wave(f, frag_s, w1_s, bias_s, out);
     if (i_raw < n)
         for (int q = 0; q < 6; ++q) y[i * 6 + q] = out[q];
 }
@@ -541,7 +589,7 @@ hipError_t alloc(T** p, size_t count) {
 void free_mlp(AtmoMlp* m) {
     void* bufs[] = {m->params, m->grads, m->adam_m, m->adam_v, m->z[0], m->z[1], m->z[2], m->z[3],
                     m->a[0], m->a[1], m->a[2], m->dz[0], m->dz[1], m->dz[2], m->dz[3], m->xbuf,
-                    m->partial, m->loss, m->frag, m->bias};
+                    m->wbuf, m->partial, m->loss, m->frag, m->bias};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
 }
@@ -603,6 +651,7 @@ int32_t atmo_mlp_create(int32_t max_batch, AtmoMlp** out) {
               alloc(&m->adam_m, ATMO_MLP_PARAMS) == hipSuccess &&
               alloc(&m->adam_v, ATMO_MLP_PARAMS) == hipSuccess &&
               alloc(&m->xbuf, B * 5) == hipSuccess &&
+              alloc(&m->wbuf, B * 6) == hipSuccess &&
               alloc(&m->partial, chunks * (64 * 64 + 64)) == hipSuccess &&
               alloc(&m->loss, B * 6 / 256 + 2) == hipSuccess &&
               alloc(&m->frag, (size_t)kFragments * 64 * 4) == hipSuccess &&
@@ -622,6 +671,8 @@ int32_t atmo_mlp_create(int32_t max_batch, AtmoMlp** out) {
         return ATMO_ENOMEM;
     }
     m->transform.altitude_max_m = 100000.0f;
+    m->transform.altitude_scale_m = 100.0f;
+    m->transform.output_mode = 1;
     m->transform.distance_min_m = 1.0f;
     m->transform.distance_max_m = 1.0e6f;
     for (int q = 0; q < 6; ++q) m->transform.out_scale[q] = 1.0f;
@@ -664,8 +715,9 @@ int32_t atmo_mlp_get_params(AtmoMlp* m, float* host_params, int64_t count) {
 
 int32_t atmo_mlp_set_transform(AtmoMlp* m, const AtmoSurrogateTransform* t) {
     if (!mlp_ok(m)) return ATMO_EHANDLE;
-    if (!t || !(t->altitude_max_m > 0.0f) || !(t->distance_min_m > 0.0f) ||
-        !(t->distance_max_m > t->distance_min_m))
+    if (!t || !(t->altitude_max_m > 0.0f) || !(t->altitude_scale_m > 0.0f) ||
+        !(t->distance_min_m > 0.0f) ||
+        !(t->distance_max_m > t->distance_min_m) || t->output_mode < 0 || t->output_mode > 1)
         return ATMO_EINVAL;
     m->transform = *t;
     return ATMO_OK;
@@ -683,8 +735,9 @@ int32_t atmo_mlp_forward(AtmoMlp* m, const float* x, int32_t B, float* y, void* 
     return status(m, "atmo_mlp_forward");
 }
 
-int32_t atmo_mlp_gradients(AtmoMlp* m, const float* x, const float* target, const int32_t* index,
-                           int32_t B, float* grad_out, float* loss_out, void* stream) {
+int32_t atmo_mlp_gradients(AtmoMlp* m, const float* x, const float* target, const float* weight,
+                           const int32_t* index, int32_t B, float* grad_out, float* loss_out,
+                           void* stream) {
     if (!mlp_ok(m)) return ATMO_EHANDLE;
     if (!x || !target || B <= 0 || B > m->max_batch) return ATMO_ESHAPE;
     hipStream_t s = (hipStream_t)stream;
@@ -693,9 +746,13 @@ int32_t atmo_mlp_gradients(AtmoMlp* m, const float* x, const float* target, cons
     // the targets are gathered into dz[3] first and replaced by the gradient
     hipLaunchKernelGGL(k_gather, dim3(blocks_for((long)B * 6, 256)), dim3(256), 0, s, B, 6, index,
                        target, m->dz[3]);
+    if (weight)
+        hipLaunchKernelGGL(k_gather, dim3(blocks_for((long)B * 6, 256)), dim3(256), 0, s, B, 6,
+                           index, weight, m->wbuf);
     forward_layers(m, B, s);
     const int lblocks = (int)blocks_for((long)B * 6, 256);
-    hipLaunchKernelGGL(k_mse, dim3(lblocks), dim3(256), 0, s, B, m->z[3], m->dz[3], m->dz[3], m->loss);
+    hipLaunchKernelGGL(k_mse, dim3(lblocks), dim3(256), 0, s, B, m->z[3], m->dz[3],
+                       weight ? (const float*)m->wbuf : nullptr, m->dz[3], m->loss);
     hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(256), 0, s, lblocks, (long)B * 6, m->loss);
     backward_layers(m, B, s);
     if (grad_out &&
@@ -706,11 +763,12 @@ int32_t atmo_mlp_gradients(AtmoMlp* m, const float* x, const float* target, cons
     return status(m, "atmo_mlp_gradients");
 }
 
-int32_t atmo_mlp_train_step(AtmoMlp* m, const float* x, const float* target, const int32_t* index,
-                            int32_t B, float lr, int32_t step, float* loss_out, void* stream) {
+int32_t atmo_mlp_train_step(AtmoMlp* m, const float* x, const float* target, const float* weight,
+                            const int32_t* index, int32_t B, float lr, int32_t step, float* loss_out,
+                            void* stream) {
     if (!mlp_ok(m)) return ATMO_EHANDLE;
     if (step < 1) return ATMO_EINVAL;
-    int32_t st = atmo_mlp_gradients(m, x, target, index, B, nullptr, loss_out, stream);
+    int32_t st = atmo_mlp_gradients(m, x, target, weight, index, B, nullptr, loss_out, stream);
     if (st != ATMO_OK) return st;
     const float b1 = 0.9f, b2 = 0.999f;
     const float bc1 = (float)(1.0 - pow(0.9, step)), bc2 = (float)(1.0 - pow(0.999, step));
@@ -731,7 +789,7 @@ int32_t atmo_mlp_infer(AtmoMlp* m, const float* x, int64_t n, float* y, void* st
     if (!mlp_ok(m)) return ATMO_EHANDLE;
     if (!x || !y || n <= 0) return ATMO_ESHAPE;
     hipLaunchKernelGGL(k_mlp_infer, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, n, x,
-                       m->frag, m->bias, y);
+                       m->frag, m->params + kLayers[0].w, m->bias, y);
     return status(m, "atmo_mlp_infer");
 }
 
@@ -796,11 +854,17 @@ int32_t atmo_composite_surrogate(AtmoHandle* h, AtmoMlp* m, const float* scene_h
     a.c0 = c0;
     a.c1 = c1;
     a.frag = m->frag;
+    a.w1 = m->params + kLayers[0].w;
     a.bias = m->bias;
     a.t = m->transform;
     const long npix = (long)h->width * h->height;
-    hipLaunchKernelGGL(k_composite_surrogate, dim3(blocks_for(npix, 256)), dim3(256), 0,
-                       (hipStream_t)stream, h->k, a);
+    const long groups = (npix + 255) / 256;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, s, h->queue);
+    hipLaunchKernelGGL(k_composite_surrogate, dim3((unsigned)(groups < 1024 ? groups : 1024)),
+                       dim3(256), 0, s, h->k, a, h->queue);
+    hipLaunchKernelGGL(k_composite_fallback, dim3(256), dim3(256), 0, s, h->k, a,
+                       (const int*)h->queue);
     return atmo_launch_status(h, "atmo_composite_surrogate");
 }
 

@@ -11,8 +11,9 @@ linear in the solar irradiance (the multi-scattering LUT is too), so a
 Everything numeric runs in libatmo.so: labels from the GPU integrator
 (atmo_surrogate_labels), FP32 forward / backward / Adam training kernels,
 BF16 MFMA inference.  This module drives them and handles the target
-transform (log of in-scatter and transmittance, standardised per output)
-and the weight file (raw float32 parameters + JSON metadata).
+transform (in-scatter and transmittance standardised per output, linear by
+default, logarithmic as an option) and the weight file (raw float32
+parameters + JSON metadata).
 """
 
 from __future__ import annotations
@@ -30,9 +31,23 @@ from . import _capi
 from ._capi import AtmoSurrogateTransform, library
 
 ALTITUDE_MAX_M = 100_000.0
+ALTITUDE_SCALE_M = 100.0
 DISTANCE_RANGE_M = (1.0, 1.0e6)
+# Loss weighting: the composite's output is E * L + scene * T, so the
+# relative error of a pixel is about w_L dlog L + w_T dlog T with the shares
+# w_L = E L / (E L + s T), w_T = s T / (E L + s T).  E is the solar
+# irradiance and s a representative scene radiance.
+WEIGHT_IRRADIANCE = (4.0, 3.9, 3.7)
+WEIGHT_SCENE = 0.5
+WEIGHT_FLOOR = 0.01
+# A pixel's radiance never counts as less than this in the relative-error
+# weights (a twilight pixel with neither in-scatter nor a lit scene).
+WEIGHT_PIXEL_FLOOR = 5.0e-3
+WEIGHT_CLIP = 1.0e3
 EPS_INSCATTER = 1.0e-8
 EPS_TRANSMITTANCE = 1.0e-6
+OUTPUT_LOG = 0
+OUTPUT_LINEAR = 1
 SEED_BASE = 20261203
 
 
@@ -43,37 +58,76 @@ def _torch():
 
 
 def sample_inputs(n: int, seed: int) -> np.ndarray:
-    """(n, 5) float32 raw samples: h ~ U[0, 100 km], mu ~ U[-1, 1],
-    mu_s ~ U[-1, 1], phi ~ U[0, pi], d ~ logU[1 m, 1000 km]."""
+    """(n, 5) float32 raw samples over the training domain: h in [0, 100 km]
+    (half U[0, 100 km], half logU[1 m, 100 km], so low cameras are as well
+    covered as high ones), mu ~ U[-1, 1], mu_s ~ U[-1, 1], phi ~ U[0, pi],
+    d ~ logU[1 m, 1000 km]."""
     g = np.random.Generator(np.random.PCG64(seed))
     lo, hi = DISTANCE_RANGE_M
     raw = np.empty((n, 5), dtype=np.float64)
-    raw[:, 0] = g.uniform(0.0, ALTITUDE_MAX_M, n)
+    half = n // 2
+    raw[:half, 0] = g.uniform(0.0, ALTITUDE_MAX_M, half)
+    raw[half:, 0] = np.exp(g.uniform(0.0, math.log(ALTITUDE_MAX_M), n - half))
     raw[:, 1] = g.uniform(-1.0, 1.0, n)
     raw[:, 2] = g.uniform(-1.0, 1.0, n)
     raw[:, 3] = g.uniform(0.0, math.pi, n)
     raw[:, 4] = np.exp(g.uniform(math.log(lo), math.log(hi), n))
-    return raw.astype(np.float32)
+    return raw[g.permutation(n)].astype(np.float32)
 
 
-def targets_from_labels(labels: np.ndarray) -> np.ndarray:
-    """log(L + eps) for the in-scatter columns, log(T + eps) for the
-    transmittance columns."""
+def targets_from_labels(labels: np.ndarray, mode: int = OUTPUT_LINEAR) -> np.ndarray:
+    """Training targets before standardisation: the labels themselves
+    (in-scatter >= 0, transmittance clamped to [0, 1]) in linear mode,
+    log(L + eps) and log(T + eps) in log mode."""
     labels = np.asarray(labels, dtype=np.float64)
     out = np.empty_like(labels)
+    if mode == OUTPUT_LINEAR:
+        out[:, :3] = np.maximum(labels[:, :3], 0.0)
+        out[:, 3:] = np.clip(labels[:, 3:], 0.0, 1.0)
+        return out
     out[:, :3] = np.log(np.maximum(labels[:, :3], 0.0) + EPS_INSCATTER)
     out[:, 3:] = np.log(np.clip(labels[:, 3:], 0.0, 1.0) + EPS_TRANSMITTANCE)
     return out
 
 
-def decode_outputs(raw: np.ndarray, mean: np.ndarray, scale: np.ndarray):
+def decode_outputs(raw: np.ndarray, mean: np.ndarray, scale: np.ndarray,
+                   mode: int = OUTPUT_LINEAR):
     """Network outputs -> (L per unit irradiance, T), the inverse of
     targets_from_labels after standardisation (host mirror of the kernel's
     decode)."""
     y = np.asarray(raw, dtype=np.float64) * scale + mean
+    if mode == OUTPUT_LINEAR:
+        return np.maximum(y[:, :3], 0.0), np.clip(y[:, 3:], 0.0, 1.0)
     L = np.maximum(np.exp(y[:, :3]) - EPS_INSCATTER, 0.0)
     T = np.clip(np.exp(y[:, 3:]) - EPS_TRANSMITTANCE, 0.0, 1.0)
     return L, T
+
+
+def loss_weights(labels: np.ndarray, scale: np.ndarray, mode: int = OUTPUT_LINEAR) -> np.ndarray:
+    """Per-output weights of the standardised MSE, normalised to mean 1.
+
+    The composite pixel of channel c is P = E_c L_c + s T_c.  In linear mode
+    an error of dy standardised units in L_c moves P by E_c sigma dy, so the
+    weight (E_c sigma / P)^2 turns the loss into the squared relative pixel
+    error (likewise s sigma / P for T_c); P is floored at
+    WEIGHT_PIXEL_FLOOR and weights are clipped to WEIGHT_CLIP x their
+    median.  In log mode the weight is (share * sigma)^2 with the shares
+    E L / P and s T / P, floored at WEIGHT_FLOOR."""
+    labels = np.asarray(labels, dtype=np.float64)
+    scale = np.asarray(scale, dtype=np.float64)
+    el = np.asarray(WEIGHT_IRRADIANCE) * np.maximum(labels[:, :3], 0.0)
+    st = WEIGHT_SCENE * np.clip(labels[:, 3:], 0.0, 1.0)
+    total = np.maximum(el + st, 1e-30)
+    if mode == OUTPUT_LINEAR:
+        pixel = np.maximum(total, WEIGHT_PIXEL_FLOOR)
+        coef = np.concatenate([np.broadcast_to(np.asarray(WEIGHT_IRRADIANCE), el.shape),
+                               np.full(st.shape, WEIGHT_SCENE)], axis=1)
+        w = (coef * scale / np.concatenate([pixel, pixel], axis=1)) ** 2
+        w = np.minimum(w, WEIGHT_CLIP * np.median(w, axis=0))
+    else:
+        share = np.concatenate([el / total, st / total], axis=1)
+        w = (np.maximum(share, WEIGHT_FLOOR) * scale) ** 2
+    return (w / w.mean()).astype(np.float32)
 
 
 def encode_host(raw: np.ndarray) -> np.ndarray:
@@ -81,7 +135,8 @@ def encode_host(raw: np.ndarray) -> np.ndarray:
     raw = np.asarray(raw, dtype=np.float64)
     lo, hi = math.log(DISTANCE_RANGE_M[0]), math.log(DISTANCE_RANGE_M[1])
     x = np.empty_like(raw)
-    x[:, 0] = raw[:, 0] * (2.0 / ALTITUDE_MAX_M) - 1.0
+    x[:, 0] = np.log1p(np.maximum(raw[:, 0], 0.0) / ALTITUDE_SCALE_M) * (
+        2.0 / math.log1p(ALTITUDE_MAX_M / ALTITUDE_SCALE_M)) - 1.0
     x[:, 1] = raw[:, 1]
     x[:, 2] = raw[:, 2]
     x[:, 3] = raw[:, 3] * (2.0 / math.pi) - 1.0
@@ -128,6 +183,7 @@ class SurrogateMLP:
         self.max_batch = int(max_batch)
         self.out_mean = np.zeros(6)
         self.out_scale = np.ones(6)
+        self.output_mode = OUTPUT_LINEAR
         self.metadata: dict = {}
         self.set_params(init_params(0))
         self._push_transform()
@@ -161,11 +217,13 @@ class SurrogateMLP:
     def transform(self) -> AtmoSurrogateTransform:
         t = AtmoSurrogateTransform()
         t.altitude_max_m = ALTITUDE_MAX_M
+        t.altitude_scale_m = ALTITUDE_SCALE_M
         t.distance_min_m, t.distance_max_m = DISTANCE_RANGE_M
         t.out_mean[:] = [float(v) for v in self.out_mean]
         t.out_scale[:] = [float(v) for v in self.out_scale]
         t.eps_inscatter = EPS_INSCATTER
         t.eps_transmittance = EPS_TRANSMITTANCE
+        t.output_mode = int(self.output_mode)
         return t
 
     def _push_transform(self) -> None:
@@ -186,9 +244,13 @@ class SurrogateMLP:
             self._handle, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.size))
         return p
 
-    def set_output_transform(self, mean, scale) -> None:
+    def set_output_transform(self, mean, scale, mode: int | None = None) -> None:
         self.out_mean = np.asarray(mean, dtype=np.float64)
         self.out_scale = np.asarray(scale, dtype=np.float64)
+        if mode is not None:
+            if mode not in (OUTPUT_LOG, OUTPUT_LINEAR):
+                raise ValueError(f"output mode must be {OUTPUT_LOG} or {OUTPUT_LINEAR}, got {mode}")
+            self.output_mode = int(mode)
         self._push_transform()
 
     # ------------------------------------------------------------- compute
@@ -207,20 +269,23 @@ class SurrogateMLP:
                                                y.data_ptr(), self._stream()))
         return y
 
-    def gradients(self, x, target, index=None, batch=None):
+    def gradients(self, x, target, index=None, batch=None, weight=None):
         torch = _torch()
         batch = int(batch if batch is not None else x.shape[0])
         grad = torch.empty(_capi.MLP_PARAMS, dtype=torch.float32, device=self.device)
         loss = torch.empty(1, dtype=torch.float32, device=self.device)
         self._check(self._lib.atmo_mlp_gradients(
             self._handle, x.data_ptr(), target.data_ptr(),
+            weight.data_ptr() if weight is not None else None,
             index.data_ptr() if index is not None else None, batch, grad.data_ptr(),
             loss.data_ptr(), self._stream()))
         return grad, loss
 
-    def train_step(self, x, target, index, batch: int, lr: float, step: int, loss_out=None):
+    def train_step(self, x, target, index, batch: int, lr: float, step: int, loss_out=None,
+                   weight=None):
         self._check(self._lib.atmo_mlp_train_step(
             self._handle, x.data_ptr(), target.data_ptr(),
+            weight.data_ptr() if weight is not None else None,
             index.data_ptr() if index is not None else None, int(batch), float(lr), int(step),
             loss_out.data_ptr() if loss_out is not None else None, self._stream()))
 
@@ -248,9 +313,11 @@ class SurrogateMLP:
             "layers": [[5, 64], [64, 64], [64, 64], [64, 6]],
             "activation": "silu",
             "params": _capi.MLP_PARAMS,
+            "output_mode": "linear" if self.output_mode == OUTPUT_LINEAR else "log",
             "out_mean": [float(v) for v in self.out_mean],
             "out_scale": [float(v) for v in self.out_scale],
             "altitude_max_m": ALTITUDE_MAX_M,
+            "altitude_scale_m": ALTITUDE_SCALE_M,
             "distance_range_m": list(DISTANCE_RANGE_M),
             "eps_inscatter": EPS_INSCATTER,
             "eps_transmittance": EPS_TRANSMITTANCE,
@@ -266,7 +333,8 @@ class SurrogateMLP:
         params = np.fromfile(path.with_suffix(".bin"), dtype="<f4")
         net = cls(max_batch=max_batch, device=device)
         net.set_params(params)
-        net.set_output_transform(meta["out_mean"], meta["out_scale"])
+        mode = OUTPUT_LINEAR if meta.get("output_mode", "log") == "linear" else OUTPUT_LOG
+        net.set_output_transform(meta["out_mean"], meta["out_scale"], mode)
         net.metadata = meta
         return net
 
@@ -292,9 +360,14 @@ def generate_labels(renderer, raw_host: np.ndarray, planet_radius_m: float,
 
 def train(renderer, planet_radius_m: float, sun_angular_radius_deg: float,
           samples: int = 1 << 22, batch: int = 65536, epochs: int = 100, lr: float = 1.0e-3,
-          seed: int = SEED_BASE + 3, log=None) -> tuple[SurrogateMLP, TrainingReport]:
+          seed: int = SEED_BASE + 3, schedule: str = "constant", lr_final: float = 1.0e-5,
+          weighted: bool = True, mode: int = OUTPUT_LINEAR,
+          log=None) -> tuple[SurrogateMLP, TrainingReport]:
     """Label `samples` seeded inputs with the GPU integrator and train the
-    MLP with Adam (constant lr) for `epochs` passes in shuffled batches."""
+    MLP with Adam for `epochs` passes in shuffled batches.  `schedule`
+    "constant" keeps lr; "cosine" anneals it from lr to lr_final.  With
+    `weighted`, each output's squared error is weighted as in loss_weights;
+    `mode` picks the linear or log output transform."""
     torch = _torch()
     device = renderer.device
     t0 = time.perf_counter()
@@ -302,16 +375,20 @@ def train(renderer, planet_radius_m: float, sun_angular_radius_deg: float,
     raw, labels = generate_labels(renderer, raw_host, planet_radius_m, sun_angular_radius_deg)
     torch.cuda.synchronize(device)
     label_s = time.perf_counter() - t0
-    targets = targets_from_labels(labels.cpu().numpy())
+    labels_host = labels.cpu().numpy()
+    targets = targets_from_labels(labels_host, mode)
     mean, scale = targets.mean(axis=0), targets.std(axis=0)
     scale = np.where(scale > 1e-6, scale, 1.0)
     t_std = torch.from_numpy(((targets - mean) / scale).astype(np.float32)).to(device)
+    weight = (torch.from_numpy(loss_weights(labels_host, scale, mode)).to(device)
+              if weighted else None)
     net = SurrogateMLP(max_batch=batch, device=device)
-    net.set_output_transform(mean, scale)
+    net.set_output_transform(mean, scale, mode)
     net.set_params(init_params(seed))
     x = net.encode(raw)
     g = np.random.Generator(np.random.PCG64(seed + 1))
     steps_per_epoch = samples // batch
+    total_steps = steps_per_epoch * epochs
     loss = torch.zeros(1, dtype=torch.float32, device=device)
     report = TrainingReport(samples, batch, epochs, 0, lr, 0.0, label_s)
     step = 0
@@ -319,8 +396,12 @@ def train(renderer, planet_radius_m: float, sun_angular_radius_deg: float,
     for epoch in range(epochs):
         perm = torch.from_numpy(g.permutation(samples).astype(np.int32)).to(device)
         for b in range(steps_per_epoch):
+            rate = lr
+            if schedule == "cosine":
+                rate = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * step / total_steps))
             step += 1
-            net.train_step(x, t_std, perm[b * batch:(b + 1) * batch], batch, lr, step, loss)
+            net.train_step(x, t_std, perm[b * batch:(b + 1) * batch], batch, rate, step, loss,
+                           weight=weight)
         if epoch % 10 == 9 or epoch == epochs - 1:
             value = float(loss.item())
             report.losses.append((epoch + 1, value))
@@ -331,7 +412,10 @@ def train(renderer, planet_radius_m: float, sun_angular_radius_deg: float,
     report.steps = step
     report.seconds = time.perf_counter() - t1
     net.metadata.update({"samples": samples, "batch": batch, "epochs": epochs, "lr": lr,
-                         "seed": seed, "planet_radius_m": planet_radius_m,
+                         "schedule": schedule, "lr_final": lr_final if schedule == "cosine" else lr,
+                         "weighted_loss": weighted, "seed": seed,
+                         "output_mode": "linear" if mode == OUTPUT_LINEAR else "log",
+                         "planet_radius_m": planet_radius_m,
                          "sun_angular_radius_deg": sun_angular_radius_deg,
                          "final_loss": report.losses[-1][1] if report.losses else None})
     return net, report
@@ -344,6 +428,7 @@ __all__ = [
     "encode_host",
     "generate_labels",
     "init_params",
+    "loss_weights",
     "sample_inputs",
     "targets_from_labels",
     "train",

@@ -57,11 +57,34 @@ def test_input_normalisation_and_target_transform(pkg):
     assert np.all(np.abs(x) <= 1.0 + 1e-6)
     assert x[:, 4].min() < -0.99 and x[:, 4].max() > 0.99  # log-uniform distance spans the range
     labels = np.column_stack([np.geomspace(1e-7, 0.5, 50)] * 3 + [np.linspace(0.0, 1.0, 50)] * 3)
-    t = s.targets_from_labels(labels)
-    mean, scale = t.mean(axis=0), t.std(axis=0)
-    L, T = s.decode_outputs((t - mean) / scale, mean, scale)
-    np.testing.assert_allclose(L, labels[:, :3], rtol=1e-9, atol=1e-15)
-    np.testing.assert_allclose(T, labels[:, 3:], rtol=1e-9, atol=1e-12)
+    for mode in (s.OUTPUT_LINEAR, s.OUTPUT_LOG):
+        t = s.targets_from_labels(labels, mode)
+        mean, scale = t.mean(axis=0), t.std(axis=0)
+        L, T = s.decode_outputs((t - mean) / scale, mean, scale, mode)
+        np.testing.assert_allclose(L, labels[:, :3], rtol=1e-9, atol=1e-15)
+        np.testing.assert_allclose(T, labels[:, 3:], rtol=1e-9, atol=1e-12)
+
+
+def test_linear_loss_weights_measure_relative_pixel_error(pkg):
+    """In linear mode, weight x (standardised error)^2 summed over L_c and
+    T_c is the squared relative error of the pixel E L + s T (before the
+    mean normalisation), as long as no floor or clip applies."""
+    import importlib
+
+    s = importlib.import_module(pkg.__name__ + ".surrogate")
+    g = np.random.default_rng(3)
+    labels = np.column_stack([g.uniform(0.01, 0.1, (64, 3)), g.uniform(0.2, 1.0, (64, 3))])
+    scale = g.uniform(0.05, 0.5, 6)
+    w = s.loss_weights(labels, scale, s.OUTPUT_LINEAR).astype(np.float64)
+    # undo the normalisation with one pixel computed by hand
+    E, sc = np.asarray(s.WEIGHT_IRRADIANCE), s.WEIGHT_SCENE
+    pixel = E * labels[:, :3] + sc * labels[:, 3:]
+    want = np.concatenate([(E * scale[:3] / pixel) ** 2, (sc * scale[3:] / pixel) ** 2], axis=1)
+    np.testing.assert_allclose(w / w[0, 0], want / want[0, 0], rtol=1e-5)
+    # a perturbation dy of output L_0 changes channel 0 by E_0 scale_0 dy
+    dy = 1e-4
+    rel = E[0] * scale[0] * dy / pixel[:, 0]
+    np.testing.assert_allclose(np.sqrt(want[:, 0]) * dy, rel, rtol=1e-12)
 
 
 def test_init_params_layout(pkg):
