@@ -134,6 +134,22 @@ int32_t atmo_get_buffer(AtmoHandle* handle, int32_t which, void** device_ptr,
 int32_t atmo_bind_buffer(AtmoHandle* handle, int32_t which, void* device_ptr,
                          int64_t bytes);
 
+/* Arithmetic of a handle's kernels.  REFERENCE (the default) follows the
+ * reference's float32 kernels operation for operation.  ROBUST has no
+ * counterpart in the reference: the same float32 march with float64
+ * evaluation where float32 cancels at planet scale -- the ray-sphere solves
+ * near tangency (renderer.py:211-301, 397-432), the sky-view horizon split
+ * (renderer.py:482-520) and the camera-relative surface endpoint above
+ * 1000 km (renderer.py:1422-1472) -- plus Kahan-compensated optical depth
+ * (renderer.py:570-607) and the segment integral (1 - e^-x) / sigma
+ * (renderer.py:806-816) through expm1.  The batched LUT builds always use
+ * REFERENCE. */
+enum {
+    ATMO_PRECISION_REFERENCE = 0,
+    ATMO_PRECISION_ROBUST = 1
+};
+int32_t atmo_set_precision(AtmoHandle* handle, int32_t precision);
+
 /* Replaces `_build_transmittance(bottom_radius)` (renderer.py:1034-1063). */
 int32_t atmo_build_transmittance(AtmoHandle* handle, float bottom_radius,
                                  void* stream);
@@ -249,16 +265,22 @@ int32_t atmo_batch_get_buffer(AtmoBatch* batch, int32_t which, void** device_ptr
 /* Parameter count: W1 64x5, b1, W2 64x64, b2, W3 64x64, b3, W4 6x64, b4. */
 #define ATMO_MLP_PARAMS 9094
 
-/* Input normalisation, training domain and output decoding.  Inputs are
- * (camera altitude h, view-zenith cosine, sun-zenith cosine, relative
- * azimuth phi, segment length d) mapped to [-1, 1]: altitude as
- * log(1 + h / altitude_scale_m) over [0, altitude_max_m], phi / pi, and
- * log d over [distance_min_m, distance_max_m], each linearly.
- * Outputs y (6) are standardised: v = y * out_scale + out_mean.  With
- * output_mode 1 (linear) L_c = max(v_c, 0) (in-scatter per unit
- * irradiance, c = 0..2) and T_c = clamp(v_{3+c}, 0, 1); with output_mode 0
- * (log) L_c = exp(v_c) - eps_inscatter and T_c = exp(v_{3+c}) -
- * eps_transmittance. */
+/* Input normalisation, training domain and output decoding.
+ * Inputs (camera altitude h, view-zenith cosine, sun-zenith cosine,
+ * relative azimuth phi, segment length d), each mapped to [-1, 1]:
+ * log(1 + h / altitude_scale_m) over [0, altitude_max_m]; the cosines raw
+ * when cosine_stretch is 0, else asinh((c - c_h) / s) / asinh(2 / s) with
+ * s = cosine_stretch and c_h the local horizon cosine at h over a planet of
+ * planet_radius_m; phi / pi; log d over [distance_min_m, distance_max_m].
+ * Outputs y (6) are standardised: v = y * out_scale + out_mean.
+ * output_mode 1 (linear): L_c = max(v_c, 0) (in-scatter per unit
+ * irradiance, c = 0..2), T_c = clamp(v_{3+c}, 0, 1).  output_mode 0 (log):
+ * L_c = exp(v_c) - eps_inscatter, T_c = exp(v_{3+c}) - eps_transmittance.
+ * output_mode 2 (floored log, trained on log(max(L, eps_inscatter))):
+ * L_c = exp(v_c), zero below 1.5 eps_inscatter; T_c as in mode 0.
+ * exact_transmittance = 1 makes the surrogate composite evaluate T with the
+ * plain two-sample formula (renderer.py:1604-1665) instead of the network's
+ * T outputs. */
 typedef struct AtmoSurrogateTransform {
     float altitude_max_m;
     float altitude_scale_m;
@@ -269,6 +291,9 @@ typedef struct AtmoSurrogateTransform {
     float eps_inscatter;
     float eps_transmittance;
     int32_t output_mode;
+    float planet_radius_m;
+    float cosine_stretch;
+    int32_t exact_transmittance;
 } AtmoSurrogateTransform;
 
 typedef struct AtmoMlp AtmoMlp;

@@ -12,7 +12,16 @@
  *               reference's `ti.init(default_fp=ti.f32)` kernels;
  *   REAL=double (libref_cpu_f64.so) -- the same approximation re-integrated
  *               in float64 (the FP64 oracle; raise step counts and LUT sizes
- *               in AtmoParams for converged quadrature).
+ *               in AtmoParams for converged quadrature);
+ *   REAL=float + ORACLE_ROBUST (libref_cpu_f32r.so) -- the float32 build
+ *               with the robust-precision islands of the GPU library's
+ *               robust mode: the ray-sphere solves (sphere_roots,
+ *               shell_roots_from_altitude, altitude_on_ray, distance_to_top)
+ *               and the transmittance-LUT cosine evaluated in float64 and
+ *               rounded once, and Kahan-compensated optical-depth sums in
+ *               integrate_extinction.  Not the reference's arithmetic: the
+ *               checker of the robust GPU path, judged against the f64
+ *               build.
  *
  * Semantic assumptions (the Taichi behaviour this restatement fixes):
  *  1. Literal constants are float32 in f32 kernels (default_fp=f32); the
@@ -56,6 +65,12 @@
 #define REAL float
 #endif
 typedef REAL real;
+
+/* Robust-precision islands: float64 evaluation inside the float32 build. */
+typedef double wide;
+#if defined(ORACLE_ROBUST) && !defined(ORACLE_F32)
+#error "ORACLE_ROBUST is a variant of the float32 build"
+#endif
 
 #ifdef ORACLE_F32
 _Static_assert(sizeof(0.5) == sizeof(float),
@@ -217,6 +232,27 @@ static inline v3 scattering(const K* k, real h) {
 /* renderer.py:211-226 */
 static inline void sphere_roots(const K* k, v3 origin, v3 ray, real radius,
                                 real* near, real* far, int* hit) {
+#ifdef ORACLE_ROBUST
+    {
+        /* float64 island: |origin - ray (origin . ray)| cancels for
+         * near-tangent rays at planet radius */
+        wide ox = origin.x, oy = origin.y, oz = origin.z;
+        wide rx = ray.x, ry = ray.y, rz = ray.z;
+        wide pr = (ox * rx + oy * ry) + oz * rz;
+        wide px = ox - rx * pr, py = oy - ry * pr, pz = oz - rz * pr;
+        wide perp = sqrt((px * px + py * py) + pz * pz);
+        wide clear = (wide)radius - perp;
+        *hit = clear >= 0.0;
+        *near = k->large;
+        *far = -k->large;
+        if (*hit) {
+            wide root = sqrt(fmax(fmax(clear, 0.0) * ((wide)radius + perp), 0.0));
+            *near = (real)(-pr - root);
+            *far = (real)(-pr + root);
+        }
+        return;
+    }
+#endif
     real projected = vdot(origin, ray);
     real perpendicular = vnorm(vsub(origin, vs(ray, projected)));
     real clearance = radius - perpendicular;
@@ -239,6 +275,15 @@ static inline real shell_rho(real altitude, real bottom) {
 
 /* renderer.py:237-262 */
 static inline real altitude_on_ray(real h0, real c, real t, real bottom) {
+#ifdef ORACLE_ROBUST
+    {
+        /* float64 island: the radial delta cancels near tangency */
+        wide a = fmax((wide)h0, 0.0), b = bottom, cc = c, tt = t;
+        wide delta = a * (2.0 * b + a) + 2.0 * (b + a) * cc * tt + tt * tt;
+        wide len = sqrt(fmax(b * b + delta, 0.0));
+        return (real)(delta / fmax(len + b, 1.0));
+    }
+#endif
     real altitude = fmax(h0, 0.0);
     real origin_radius = bottom + altitude;
     real radial_delta = altitude * (2.0 * bottom + altitude)
@@ -251,6 +296,30 @@ static inline real altitude_on_ray(real h0, real c, real t, real bottom) {
 static inline void shell_roots_from_altitude(const K* k, real h0, real c,
                                              real shell, real bottom,
                                              real* near, real* far, int* hit) {
+#ifdef ORACLE_ROBUST
+    {
+        /* float64 island: the discriminant cancels near tangency */
+        wide b = bottom, h = h0, sh = shell;
+        wide half_linear = (b + h) * (wide)c;
+        wide constant = (h - sh) * (2.0 * b + h + sh);
+        wide disc = half_linear * half_linear - constant;
+        *hit = disc >= 0.0;
+        *near = k->large;
+        *far = -k->large;
+        if (*hit) {
+            wide root = sqrt(fmax(disc, 0.0));
+            wide q = -half_linear - (half_linear < 0.0 ? -1.0 : 1.0) * root;
+            wide r0 = -half_linear - root, r1 = -half_linear + root;
+            if (fabs(q) > 1.0e-12) {
+                r0 = q;
+                r1 = constant / q;
+            }
+            *near = (real)fmin(r0, r1);
+            *far = (real)fmax(r0, r1);
+        }
+        return;
+    }
+#endif
     real origin_radius = bottom + h0;
     real half_linear = origin_radius * c;
     real constant = (h0 - shell) * (2.0 * bottom + h0 + shell);
@@ -343,6 +412,15 @@ static inline void aerial_prefix_segment(const K* k, real h, real c,
 
 /* renderer.py:397-408 */
 static inline real distance_to_top(const K* k, real h, real c, real bottom) {
+#ifdef ORACLE_ROBUST
+    {
+        /* float64 island: -r c + sqrt(r^2 c^2 + delta) cancels for c > 0 */
+        wide radius = (wide)bottom + fmax((wide)h, 0.0), cc = c, top = k->top;
+        wide delta = (top - (wide)h) * (2.0 * (wide)bottom + top + (wide)h);
+        wide disc = radius * radius * cc * cc + delta;
+        return (real)fmax(-radius * cc + sqrt(fmax(disc, 0.0)), 0.0);
+    }
+#endif
     real radius = bottom + fmax(h, 0.0);
     real shell_delta = (k->top - h) * (2.0 * bottom + k->top + h);
     real discriminant = radius * radius * c * c + shell_delta;
@@ -363,8 +441,15 @@ static inline void transmittance_uv_to_ray(const K* k, real u, real v,
     real dmax = rho + shell_horizon;
     real d = dmin + clampr(u, 0.0, 1.0) * (dmax - dmin);
     real cs = 1.0;
+#ifdef ORACLE_ROBUST
+    /* float64 island: the numerator cancels near the horizon */
+    if (d > 1.0e-6)
+        cs = (real)((((wide)top - alt) * (2.0 * (wide)bottom + top + alt) - (wide)d * d) /
+                    (2.0 * (wide)radius * d));
+#else
     if (d > 1.0e-6)
         cs = ((top - alt) * (2.0 * bottom + top + alt) - d * d) / (2.0 * radius * d);
+#endif
     *altitude = alt;
     *cosine = clampr(cs, -1.0, 1.0);
     *distance = d;
@@ -449,6 +534,42 @@ static inline real sky_zenith_to_v(const K* k, real zenith, real h, real bottom)
     return v;
 }
 
+#ifdef ORACLE_ROBUST
+/* float64 island: the sky-view horizon split.  Near the horizon the view
+ * zenith and the horizon zenith agree to ~1e-6 rad, and the square-root
+ * concentration amplifies their float32 difference. */
+static inline wide sky_horizon_zenith_w(const K* k, real h, real bottom) {
+    wide a = fmax((wide)h, 0.0), b = bottom;
+    wide rho = sqrt(a * (2.0 * b + a));
+    return acos((wide)-1.0) - acos(fmin(fmax(rho / fmax(b + a, 1.0), 0.0), 1.0));
+}
+
+static inline real sky_zenith_to_v_w(const K* k, wide zenith, real h, real bottom) {
+    wide pi = acos((wide)-1.0);
+    wide horizon = sky_horizon_zenith_w(k, h, bottom);
+    wide v;
+    if (zenith < horizon) {
+        wide n = fmin(fmax(zenith / fmax(horizon, 1.0e-6), 0.0), 1.0);
+        v = 0.5 * (1.0 - sqrt(fmax(1.0 - n, 0.0)));
+    } else {
+        wide n = fmin(fmax((zenith - horizon) / fmax(pi - horizon, 1.0e-6), 0.0), 1.0);
+        v = 0.5 + 0.5 * sqrt(n);
+    }
+    return (real)v;
+}
+
+static inline wide sky_v_to_zenith_w(const K* k, real v, real h, real bottom) {
+    wide pi = acos((wide)-1.0);
+    wide horizon = sky_horizon_zenith_w(k, h, bottom);
+    if (v < 0.5) {
+        wide inverse = 1.0 - 2.0 * (wide)v;
+        return horizon * (1.0 - inverse * inverse);
+    }
+    wide local = 2.0 * (wide)v - 1.0;
+    return horizon + (pi - horizon) * local * local;
+}
+#endif
+
 /* renderer.py:522-542 */
 static inline real sun_u_to_cosine(real u, real h, real bottom) {
     real radius = bottom + fmax(h, 0.0);
@@ -488,9 +609,15 @@ static inline real sun_cosine_to_u(real cosine, real h, real bottom) {
 /* ------------------------------------------------------------------ */
 
 /* renderer.py:570-607 */
+/* origin_altitude and origin_cosine (the ray's cosine to the origin's
+ * local vertical) are used only by the robust build's float64 altitude. */
 static inline v3 integrate_extinction(const K* k, v3 origin, v3 ray, real start,
-                                      real end, real bottom) {
+                                      real end, real bottom, real origin_altitude,
+                                      real origin_cosine) {
     v3 optical_depth = V(0.0, 0.0, 0.0);
+#ifdef ORACLE_ROBUST
+    v3 carry = V(0.0, 0.0, 0.0); /* Kahan compensation */
+#endif
     for (int index = 0; index < k->t_steps; ++index) {
         real f0 = (real)index / (real)k->t_steps;
         real f1 = (real)(index + 1) / (real)k->t_steps;
@@ -498,8 +625,22 @@ static inline v3 integrate_extinction(const K* k, v3 origin, v3 ray, real start,
         real d1 = warped_boundary(origin, ray, start, end, f1);
         real step = d1 - d0;
         real distance = (d0 + d1) * 0.5;
+#ifdef ORACLE_ROBUST
+        /* float64 island: |origin + ray d| - R cancels at planet scale (and
+         * the float32 origin radius R + h is itself rounded): the stable
+         * altitude along the ray from the origin's altitude, in float64 */
+        real altitude = altitude_on_ray(origin_altitude, origin_cosine, distance, bottom);
+#else
         real altitude = vnorm(vadd(origin, vs(ray, distance))) - bottom;
+#endif
+#ifdef ORACLE_ROBUST
+        v3 y = vsub(vs(extinction(k, altitude), step), carry);
+        v3 t = vadd(optical_depth, y);
+        carry = vsub(vsub(t, optical_depth), y);
+        optical_depth = t;
+#else
         optical_depth = vadd(optical_depth, vs(extinction(k, altitude), step));
+#endif
     }
     return vexp(vneg(vmins(optical_depth, 80.0)));
 }
@@ -613,10 +754,17 @@ static inline void segment_integral(v3 ext, real step, v3* transmission,
     *transmission = vexp(vneg(vmins(vs(ext, step), 80.0)));
     real e[3] = {ext.x, ext.y, ext.z};
     real t[3] = {transmission->x, transmission->y, transmission->z};
+    (void)t;
     real out[3];
     for (int c = 0; c < 3; ++c) {
         out[c] = step;
+#ifdef ORACLE_ROBUST
+        /* 1 - exp(-x) cancels for small x: -expm1(-x), rounded once */
+        if (e[c] > 1.0e-12)
+            out[c] = (real)(-expm1(-(wide)fmin(e[c] * step, 80.0))) / e[c];
+#else
         if (e[c] > 1.0e-12) out[c] = (1.0 - t[c]) / e[c];
+#endif
     }
     *integral = V(out[0], out[1], out[2]);
 }
@@ -832,7 +980,9 @@ static inline v3 camera_transmittance(const K* k, const real* tlut, real h,
 /* renderer.py:1474-1525 */
 static inline v3 sample_sky_view(const K* k, const real* svlut, v3 ray, v3 sun,
                                  real h, real bottom) {
+#ifndef ORACLE_ROBUST
     real zenith = cr_acos(clampr(ray.y, -1.0, 1.0));
+#endif
     real vhx = ray.x, vhz = ray.z, shx = sun.x, shz = sun.z;
     real view_length = sqrt(vhx * vhx + vhz * vhz);
     real sun_length = sqrt(shx * shx + shz * shz);
@@ -842,7 +992,11 @@ static inline v3 sample_sky_view(const K* k, const real* svlut, v3 ray, v3 sun,
                                            / (view_length * sun_length),
                                        -1.0, 1.0));
     real u = relative_azimuth / k->pi;
+#ifdef ORACLE_ROBUST
+    real v = sky_zenith_to_v_w(k, acos(fmin(fmax((wide)ray.y, -1.0), 1.0)), h, bottom);
+#else
     real v = sky_zenith_to_v(k, zenith, h, bottom);
+#endif
     return bilinear(svlut, k->sw, k->sh, u, v);
 }
 
@@ -920,7 +1074,7 @@ EXPORT int oracle_build_transmittance(const AtmoParams* p, real bottom,
         real radius = bottom + altitude;
         v3 origin = V(0.0, radius, 0.0);
         v3 ray = V(sine, cosine, 0.0);
-        v3 t = integrate_extinction(&k, origin, ray, 0.0, distance, bottom);
+        v3 t = integrate_extinction(&k, origin, ray, 0.0, distance, bottom, altitude, cosine);
         vstore(tlut, i, vclamp(t, 0.0, 1.0));
     }
     return ATMO_OK;
@@ -1022,9 +1176,15 @@ EXPORT int oracle_build_sky_view(const AtmoParams* p, real bottom,
         int x = (int)(i / k.sh), y = (int)(i % k.sh);
         real azimuth = ((real)x + 0.5) / (real)k.sw * k.pi;
         real view_v = ((real)y + 0.5) / (real)k.sh;
+#ifdef ORACLE_ROBUST
+        wide zenith_w = sky_v_to_zenith_w(&k, view_v, safe_alt, bottom);
+        real vc = (real)cos(zenith_w);
+        real vsn = (real)sin(zenith_w);
+#else
         real zenith = sky_v_to_zenith(&k, view_v, safe_alt, bottom);
         real vc = cr_cos(zenith);
         real vsn = cr_sin(zenith);
+#endif
         v3 ray = V(vsn * cr_cos(azimuth), vc, vsn * cr_sin(azimuth));
         real start, end;
         int valid, ground;
@@ -1144,9 +1304,23 @@ EXPORT int oracle_prepare_surface_lighting(
             v3 radial = vdivs(planet, fmax(radius, 1.0));
             real surface_distance = vnorm(local);
             v3 local_ray = vdivs(local, fmax(surface_distance, 1.0));
+            real sun_cosine = vdot(radial, sun);
+#ifdef ORACLE_ROBUST
+            if (camera_altitude > 1.0e6) {
+                /* float64 island: camera-relative endpoint above 1000 km */
+                wide lx = ((wide)right.x * vp.x + (wide)up.x * vp.y) + (wide)forward.x * vp.z;
+                wide ly = ((wide)right.y * vp.x + (wide)up.y * vp.y) + (wide)forward.y * vp.z;
+                wide lz = ((wide)right.z * vp.x + (wide)up.z * vp.y) + (wide)forward.z * vp.z;
+                wide py = ((wide)bottom + camera_altitude) + ly;
+                wide pn = fmax(sqrt((lx * lx + py * py) + lz * lz), 1.0);
+                sun_cosine = (real)(((lx * sun.x + py * sun.y) + lz * sun.z) / pn);
+                wide ld = sqrt((lx * lx + ly * ly) + lz * lz);
+                surface_distance = (real)ld;
+                local_ray.y = (real)(ly / fmax(ld, 1.0));
+            }
+#endif
             real altitude = altitude_on_ray(camera_altitude, local_ray.y,
                                             surface_distance, bottom);
-            real sun_cosine = vdot(radial, sun);
             solar = sample_solar_transmittance(&k, tlut, altitude, sun_cosine, bottom,
                                                angular_radius);
             sky = sample_multi_scattering(&k, mslut, altitude, sun_cosine, bottom);
@@ -1402,7 +1576,12 @@ EXPORT void oracle_fn_integrate_extinction(const AtmoParams* p,
                                            const real ray[3], real start,
                                            real end, real bottom, real out[3]) {
     K k = make_consts(p);
-    v3 t = integrate_extinction(&k, varr(origin), varr(ray), start, end, bottom);
+    v3 o = varr(origin), r = varr(ray);
+    wide ox = o.x, oy = o.y, oz = o.z;
+    wide radius = sqrt((ox * ox + oy * oy) + oz * oz);
+    real altitude = (real)(radius - (wide)bottom);
+    real cosine = (real)(((ox * r.x + oy * r.y) + oz * r.z) / fmax(radius, 1.0));
+    v3 t = integrate_extinction(&k, o, r, start, end, bottom, altitude, cosine);
     out[0] = t.x; out[1] = t.y; out[2] = t.z;
 }
 

@@ -3,7 +3,10 @@ ONLY -- imported by tests/, the smoke check and the CPU-baseline leg of
 bench.py as the checker, never by the shipped package.
 
 `RefCpu("f32")` is the float32 op-for-op restatement of the reference's
-Taichi kernels; `RefCpu("f64")` the same approximation in float64.
+Taichi kernels; `RefCpu("f64")` the same approximation in float64;
+`RefCpu("f32r")` the float32 build with the robust-precision islands
+(float64 ray-sphere solves, Kahan optical depth), the checker of the GPU
+library's robust mode.
 `frame()` restates the host side of one reference frame
 (src/planet_renderer/atmosphere/renderer.py:2072-2238): the argument
 derivation and float32 rounding of `update` (forcing T, MS and SV
@@ -21,7 +24,8 @@ from pathlib import Path
 import numpy as np
 
 ORACLE_DIR = Path(__file__).resolve().parent
-LIBS = {"f32": ORACLE_DIR / "libref_cpu_f32.so", "f64": ORACLE_DIR / "libref_cpu_f64.so"}
+LIBS = {"f32": ORACLE_DIR / "libref_cpu_f32.so", "f64": ORACLE_DIR / "libref_cpu_f64.so",
+        "f32r": ORACLE_DIR / "libref_cpu_f32r.so"}
 
 
 class OracleParams(ctypes.Structure):
@@ -85,8 +89,9 @@ class RefCpu:
             raise RuntimeError(f"{path} missing: run `make -C oracle`")
         self.precision = precision
         self.lib = ctypes.CDLL(str(path))
-        self.dtype = np.float32 if precision == "f32" else np.float64
-        self.real = ctypes.c_float if precision == "f32" else ctypes.c_double
+        single = precision in ("f32", "f32r")
+        self.dtype = np.float32 if single else np.float64
+        self.real = ctypes.c_float if single else ctypes.c_double
         assert self.lib.oracle_real_bytes() == np.dtype(self.dtype).itemsize
 
     # ---------------------------------------------------------------- utils

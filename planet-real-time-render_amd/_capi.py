@@ -26,6 +26,9 @@ BUF_AERIAL_SCATTERING = 3
 BUF_SURFACE_SUN_T = 4
 BUF_SURFACE_SKY_L = 5
 
+PRECISION_REFERENCE = 0
+PRECISION_ROBUST = 1
+
 # Every symbol include/atmo.h declares, checked by the CPU test suite.
 EXPORTED_SYMBOLS = (
     "atmo_validate_params",
@@ -34,6 +37,7 @@ EXPORTED_SYMBOLS = (
     "atmo_last_error",
     "atmo_get_buffer",
     "atmo_bind_buffer",
+    "atmo_set_precision",
     "atmo_build_transmittance",
     "atmo_build_multi_scattering",
     "atmo_build_sky_view",
@@ -77,6 +81,9 @@ class AtmoSurrogateTransform(ctypes.Structure):
         ("eps_inscatter", ctypes.c_float),
         ("eps_transmittance", ctypes.c_float),
         ("output_mode", ctypes.c_int32),
+        ("planet_radius_m", ctypes.c_float),
+        ("cosine_stretch", ctypes.c_float),
+        ("exact_transmittance", ctypes.c_int32),
     ]
 
 
@@ -147,6 +154,7 @@ def _signatures(lib: ctypes.CDLL) -> None:
     lib.atmo_last_error.restype = ctypes.c_char_p
     lib.atmo_get_buffer.argtypes = [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int64)]
     lib.atmo_bind_buffer.argtypes = [_P, _I, _P, ctypes.c_int64]
+    lib.atmo_set_precision.argtypes = [_P, _I]
     lib.atmo_build_transmittance.argtypes = [_P, _F, _P]
     lib.atmo_build_multi_scattering.argtypes = [_P, _F, _PF3, _F, _P]
     lib.atmo_build_sky_view.argtypes = [_P, _F, _F, _F, _PF3, _F, _P]

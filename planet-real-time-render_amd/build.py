@@ -23,7 +23,8 @@ CSRC = PACKAGE_DIR / "csrc"
 # library -> sources: libatmo.so is the product; libatmo_probe.so holds the
 # instruction-throughput probe the benchmark uses for its roofline peak.
 TARGETS = {
-    PACKAGE_DIR / "libatmo.so": [CSRC / "atmo_kernels.hip", CSRC / "surrogate.hip"],
+    PACKAGE_DIR / "libatmo.so": [CSRC / "atmo_kernels.hip", CSRC / "atmo_robust.hip",
+                                 CSRC / "surrogate.hip"],
     PACKAGE_DIR / "libatmo_probe.so": [CSRC / "probe.hip"],
 }
 ARCH = "gfx950"

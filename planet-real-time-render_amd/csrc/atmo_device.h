@@ -18,6 +18,17 @@
 //
 // Frame- and LUT-sized arrays use the reference's Taichi field layout:
 // (X, Y[, Z], 3) float32, element [x, y] at (x * Y + y) * 3.
+//
+// Two precision variants of the model functions exist, each in its own
+// inline namespace selected by ATMO_ROBUST when a translation unit includes
+// this header: atmo::ref (ATMO_ROBUST 0, the default) follows the reference
+// operation for operation in float32; atmo::robust (ATMO_ROBUST 1) is the
+// same float32 march with float64 islands where float32 cancels at planet
+// scale -- the ray-sphere solves near tangency (sphere_roots, shell_roots,
+// altitude_on_ray, distance_to_top, the transmittance-LUT cosine), the
+// sky-view horizon split and the surface endpoint above 1000 km -- plus
+// Kahan-compensated optical depth and an expm1-based segment integral.
+// Types (Consts, f3) are shared by both.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -85,6 +96,20 @@ __device__ __forceinline__ void st3(float* __restrict__ base, long i, f3 v) {
     base[i * 3 + 2] = v.z;
 }
 
+}  // namespace atmo
+
+#ifndef ATMO_ROBUST
+#define ATMO_ROBUST 0
+#endif
+#if ATMO_ROBUST
+#define ATMO_PRECISION robust
+#else
+#define ATMO_PRECISION ref
+#endif
+
+namespace atmo {
+inline namespace ATMO_PRECISION {
+
 // ---------------------------------------------------------------- media
 
 // renderer.py:168-192
@@ -115,6 +140,27 @@ __device__ __forceinline__ f3 scattering_of(const Consts& k, f3 d) {
 // renderer.py:211-226
 __device__ __forceinline__ void sphere_roots(const Consts& k, f3 origin, f3 ray, float radius,
                                              float& near, float& far, bool& hit) {
+#if ATMO_ROBUST
+    {
+        // float64 island: |origin - ray (origin . ray)| cancels for
+        // near-tangent rays at planet radius
+        const double ox = origin.x, oy = origin.y, oz = origin.z;
+        const double rx = ray.x, ry = ray.y, rz = ray.z;
+        const double pr = (ox * rx + oy * ry) + oz * rz;
+        const double px = ox - rx * pr, py = oy - ry * pr, pz = oz - rz * pr;
+        const double perp = sqrt((px * px + py * py) + pz * pz);
+        const double clear = (double)radius - perp;
+        hit = clear >= 0.0;
+        near = k.large;
+        far = -k.large;
+        if (hit) {
+            const double root = sqrt(fmax(fmax(clear, 0.0) * ((double)radius + perp), 0.0));
+            near = (float)(-pr - root);
+            far = (float)(-pr + root);
+        }
+        return;
+    }
+#endif
     float projected = dot(origin, ray);
     float perpendicular = norm(origin - ray * projected);
     float clearance = radius - perpendicular;
@@ -137,6 +183,15 @@ __device__ __forceinline__ float shell_rho(float altitude, float bottom) {
 
 // renderer.py:237-262
 __device__ __forceinline__ float altitude_on_ray(float h0, float c, float t, float bottom) {
+#if ATMO_ROBUST
+    {
+        // float64 island: the radial delta cancels near tangency
+        const double a = fmax((double)h0, 0.0), b = bottom, cc = c, tt = t;
+        const double delta = a * (2.0 * b + a) + 2.0 * (b + a) * cc * tt + tt * tt;
+        const double len = sqrt(fmax(b * b + delta, 0.0));
+        return (float)(delta / fmax(len + b, 1.0));
+    }
+#endif
     float a = fmaxf(h0, 0.0f);
     float r0 = bottom + a;
     float delta = a * (2.0f * bottom + a) + 2.0f * r0 * c * t + t * t;
@@ -147,6 +202,30 @@ __device__ __forceinline__ float altitude_on_ray(float h0, float c, float t, flo
 // renderer.py:264-301
 __device__ __forceinline__ void shell_roots(const Consts& k, float h0, float c, float shell,
                                             float bottom, float& near, float& far, bool& hit) {
+#if ATMO_ROBUST
+    {
+        // float64 island: the discriminant cancels near tangency
+        const double b = bottom, h = h0, sh = shell;
+        const double hl = (b + h) * (double)c;
+        const double constant = (h - sh) * (2.0 * b + h + sh);
+        const double disc = hl * hl - constant;
+        hit = disc >= 0.0;
+        near = k.large;
+        far = -k.large;
+        if (hit) {
+            const double root = sqrt(fmax(disc, 0.0));
+            const double q = -hl - (hl < 0.0 ? -1.0 : 1.0) * root;
+            double r0 = -hl - root, r1 = -hl + root;
+            if (fabs(q) > (double)1.0e-12f) {
+                r0 = q;
+                r1 = constant / q;
+            }
+            near = (float)fmin(r0, r1);
+            far = (float)fmax(r0, r1);
+        }
+        return;
+    }
+#endif
     float r0 = bottom + h0;
     float hl = r0 * c;
     float constant = (h0 - shell) * (2.0f * bottom + h0 + shell);
@@ -235,6 +314,15 @@ __device__ __forceinline__ void aerial_prefix(const Consts& k, float h, float c,
 
 // renderer.py:397-408
 __device__ __forceinline__ float distance_to_top(const Consts& k, float h, float c, float bottom) {
+#if ATMO_ROBUST
+    {
+        // float64 island: -r c + sqrt(r^2 c^2 + delta) cancels for c > 0
+        const double radius = (double)bottom + fmax((double)h, 0.0), cc = c, top = k.top;
+        const double delta = (top - (double)h) * (2.0 * (double)bottom + top + (double)h);
+        const double disc = radius * radius * cc * cc + delta;
+        return (float)fmax(-radius * cc + sqrt(fmax(disc, 0.0)), 0.0);
+    }
+#endif
     float radius = bottom + fmaxf(h, 0.0f);
     float shell_delta = (k.top - h) * (2.0f * bottom + k.top + h);
     float disc = radius * radius * c * c + shell_delta;
@@ -255,7 +343,14 @@ __device__ __forceinline__ void transmittance_uv_to_ray(const Consts& k, float u
     float dmax = rho + sh;
     float d = dmin + clampf(u, 0.0f, 1.0f) * (dmax - dmin);
     float cs = 1.0f;
+#if ATMO_ROBUST
+    // float64 island: the numerator cancels near the horizon
+    if (d > 1.0e-6f)
+        cs = (float)((((double)top - alt) * (2.0 * (double)bottom + top + alt) - (double)d * d) /
+                     (2.0 * (double)radius * d));
+#else
     if (d > 1.0e-6f) cs = ((top - alt) * (2.0f * bottom + top + alt) - d * d) / (2.0f * radius * d);
+#endif
     altitude = alt;
     cosine = clampf(cs, -1.0f, 1.0f);
     distance = d;
@@ -343,6 +438,40 @@ __device__ __forceinline__ float sky_zenith_to_v(const Consts& k, float zenith, 
     float n = clampf((zenith - horizon) / fmaxf(k.pi - horizon, 1.0e-6f), 0.0f, 1.0f);
     return 0.5f + 0.5f * sqrtf(n);
 }
+
+#if ATMO_ROBUST
+// float64 island: the sky-view horizon split.  Near the horizon the view
+// zenith and the horizon zenith agree to ~1e-6 rad, and the square-root
+// concentration amplifies their float32 difference.
+__device__ __forceinline__ double sky_horizon_zenith_w(float h, float bottom) {
+    const double a = fmax((double)h, 0.0), b = bottom;
+    const double rho = sqrt(a * (2.0 * b + a));
+    return 3.14159265358979323846 - acos(fmin(fmax(rho / fmax(b + a, 1.0), 0.0), 1.0));
+}
+
+__device__ __forceinline__ float sky_zenith_to_v_w(double zenith, float h, float bottom) {
+    const double horizon = sky_horizon_zenith_w(h, bottom);
+    double v;
+    if (zenith < horizon) {
+        const double n = fmin(fmax(zenith / fmax(horizon, 1.0e-6), 0.0), 1.0);
+        v = 0.5 * (1.0 - sqrt(fmax(1.0 - n, 0.0)));
+    } else {
+        const double n = fmin(fmax((zenith - horizon) / fmax(3.14159265358979323846 - horizon, 1.0e-6), 0.0), 1.0);
+        v = 0.5 + 0.5 * sqrt(n);
+    }
+    return (float)v;
+}
+
+__device__ __forceinline__ double sky_v_to_zenith_w(float v, float h, float bottom) {
+    const double horizon = sky_horizon_zenith_w(h, bottom);
+    if (v < 0.5f) {
+        const double inv = 1.0 - 2.0 * (double)v;
+        return horizon * (1.0 - inv * inv);
+    }
+    const double local = 2.0 * (double)v - 1.0;
+    return horizon + (3.14159265358979323846 - horizon) * local * local;
+}
+#endif
 
 // renderer.py:522-542
 __device__ __forceinline__ float sun_u_to_cosine(float u, float h, float bottom) {
@@ -514,9 +643,16 @@ __device__ __forceinline__ float exp_neg_segment(float x) {
 __device__ __forceinline__ void segment_integral(f3 ext, float step, f3& trans, f3& integ) {
     f3 x = mins(ext * step, 80.0f);
     trans = f3{exp_neg_segment(x.x), exp_neg_segment(x.y), exp_neg_segment(x.z)};
+#if ATMO_ROBUST
+    // 1 - exp(-x) cancels for small x: -expm1(-x) in float64, rounded once
+    integ.x = ext.x > 1.0e-12f ? (float)(-expm1(-(double)x.x)) / ext.x : step;
+    integ.y = ext.y > 1.0e-12f ? (float)(-expm1(-(double)x.y)) / ext.y : step;
+    integ.z = ext.z > 1.0e-12f ? (float)(-expm1(-(double)x.z)) / ext.z : step;
+#else
     integ.x = ext.x > 1.0e-12f ? (1.0f - trans.x) / ext.x : step;
     integ.y = ext.y > 1.0e-12f ? (1.0f - trans.y) / ext.y : step;
     integ.z = ext.z > 1.0e-12f ? (1.0f - trans.z) / ext.z : step;
+#endif
 }
 
 // Phase-weighted scattering coefficients of one ray: the view/sun cosine is
@@ -554,10 +690,6 @@ __device__ __forceinline__ f3 scattering_source(const Consts& k, const float* __
     }
     return source;
 }
-
-}  // namespace atmo
-
-namespace atmo {
 
 // One constant-source segment (renderer.py:858-899): returns L and T.
 // `oc` is origin_radial . ray for the ray's origin on the local radial axis.
@@ -724,7 +856,12 @@ __device__ __forceinline__ f3 sample_sky_view(const Consts& k, const float* __re
     if (vl > 1.0e-6f && sl > 1.0e-6f)
         rel = cr_acos(clampf((ray.x * sun.x + ray.z * sun.z) / (vl * sl), -1.0f, 1.0f));
     float u = rel / k.pi;
+#if ATMO_ROBUST
+    (void)zenith;
+    float v = sky_zenith_to_v_w(acos(fmin(fmax((double)ray.y, -1.0), 1.0)), h, bottom);
+#else
     float v = sky_zenith_to_v(k, zenith, sky_horizon_zenith(k, h, bottom));
+#endif
     return bilinear(svlut, k.sw, k.sh, u, v);
 }
 
@@ -807,4 +944,5 @@ __device__ __forceinline__ int compact_block(bool flag, int* list, int* wave_cou
     return total;
 }
 
+}  // inline namespace ATMO_PRECISION
 }  // namespace atmo

@@ -26,6 +26,7 @@ struct AtmoHandle {
     float* sky_l;
     bool borrowed[6];  // buffer supplied by the caller through atmo_bind_buffer
     int* queue;        // surrogate composite: [0] = count, then out-of-domain pixel indices
+    int precision;     // ATMO_PRECISION_REFERENCE or ATMO_PRECISION_ROBUST
     char error[512];
 };
 
@@ -66,8 +67,12 @@ struct BatchState {
     float thf;
 };
 
-// Launches the sky pass of the COMPOSITE view (atmo_kernels.hip).
+// Launch the sky pass of the COMPOSITE view, and the plain surface branch
+// for the pixels listed in list[1 .. list[0]], in the handle's precision
+// variant (atmo_kernels.hip).
 int32_t atmo_launch_composite_sky(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream);
+int32_t atmo_launch_composite_list(AtmoHandle* h, const CompositeArgs& a, const int* list,
+                                   hipStream_t stream);
 
 inline atmo::f3 mk3(const float* a) { return atmo::f3{a[0], a[1], a[2]}; }
 

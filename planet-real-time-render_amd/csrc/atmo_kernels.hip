@@ -1,19 +1,8 @@
-// atmo_kernels.hip -- HIP kernels for gfx950 and the C-ABI of libatmo.so.
-//
-// One kernel per Taichi kernel of the reference's AtmosphereRenderer
-// (src/planet_renderer/atmosphere/renderer.py:1034-2055).  Launch shapes:
-//  - transmittance / sky-view: a group of G lanes per texel; each lane
-//    marches a contiguous chunk of the texel's steps and the group combines
-//    the chunks (a sum of optical depth for T, an ordered (L, T) scan for
-//    sky-view).  A single-state LUT has only 16-20 K texels, so one thread
-//    per texel would leave most of the 256 CUs idle and expose the full
-//    serial march latency.
-//  - multi-scattering: one wave64 per texel, lane = sphere direction, the
-//    64-direction integral reduced with cross-lane shuffles.
-//  - aerial perspective: one wave64 per froxel ray, lane = depth slice; the
-//    cumulative in-scatter along the ray is an ordered (L, T) scan.
-//  - prepare / composite: one thread per pixel in the (W, H) field order,
-//    so consecutive lanes read and write consecutive 12-byte elements.
+// atmo_kernels.hip -- the C-ABI of libatmo.so (include/atmo.h) and the
+// reference-precision kernels (atmo_stages.h compiled for atmo::ref).  Each
+// entry point validates its arguments and launches the stage of the
+// handle's precision variant: atmo::ref here, atmo::robust from
+// atmo_robust.hip.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -24,10 +13,19 @@
 #include <new>
 
 #include "../../include/atmo.h"
-#include "atmo_device.h"
-#include "atmo_handle.h"
+#include "atmo_stages.h"
+#include "atmo_launch.h"
 
 using namespace atmo;
+
+// Calls the launcher `fn` of the handle's precision variant.
+#define ATMO_STAGE(h, fn, ...)                                              \
+    do {                                                                    \
+        if ((h)->precision == ATMO_PRECISION_ROBUST)                        \
+            atmo::robust::fn(__VA_ARGS__);                                  \
+        else                                                                \
+            atmo::ref::fn(__VA_ARGS__);                                     \
+    } while (0)
 
 namespace {
 
@@ -151,521 +149,13 @@ Consts make_consts(const AtmoParams& p) {
 
 f3 host3(const float* a) { return f3{a[0], a[1], a[2]}; }
 
-// Ordered combination of consecutive (L, T) segments within a lane group of
-// width G: afterwards lane i holds the composition of segments 0..i
-// (earlier segment first: L = L' + T' * L, T = T' * T).
-template <int G>
-__device__ __forceinline__ void scan_segments(f3& L, f3& T, int lane) {
-#pragma unroll
-    for (int off = 1; off < G; off <<= 1) {
-        f3 pl = mk(__shfl_up(L.x, off, G), __shfl_up(L.y, off, G), __shfl_up(L.z, off, G));
-        f3 pt = mk(__shfl_up(T.x, off, G), __shfl_up(T.y, off, G), __shfl_up(T.z, off, G));
-        if (lane >= off) {
-            L = pl + pt * L;
-            T = pt * T;
-        }
-    }
-}
-
 }  // namespace
-
-// ------------------------------------------------------------ kernels
-
-// _build_transmittance (renderer.py:1034-1063) with _integrate_extinction
-// (renderer.py:570-607).  G lanes per texel.
-template <int G>
-__device__ __forceinline__ void transmittance_body(const Consts& k, float bottom,
-                                                   float* __restrict__ tlut, long gid) {
-    const int lane = threadIdx.x % G;
-    const long n = (long)k.tw * k.th;
-    const long texel_raw = gid / G;
-    const bool active = texel_raw < n;
-    const long texel = active ? texel_raw : n - 1;
-    const int x = (int)(texel / k.th), y = (int)(texel % k.th);
-    float u = ((float)x + 0.5f) / (float)k.tw;
-    float v = ((float)y + 0.5f) / (float)k.th;
-    float alt, c, distance;
-    transmittance_uv_to_ray(k, u, v, bottom, alt, c, distance);
-    float sine = sqrtf(fmaxf(1.0f - c * c, 0.0f));
-    f3 origin = mk(0.0f, bottom + alt, 0.0f);
-    f3 ray = mk(sine, c, 0.0f);
-    Warp w = make_warp(-dot(origin, ray), 0.0f, distance);
-    const int steps = k.t_steps;
-    const int per = (steps + G - 1) / G;
-    const int s0 = min(lane * per, steps), s1 = min(s0 + per, steps);
-    f3 od = mk(0.0f, 0.0f, 0.0f);
-    for (int s = s0; s < s1; ++s) {
-        float d0 = boundary(w, (float)s / (float)steps);
-        float d1 = boundary(w, (float)(s + 1) / (float)steps);
-        float step = d1 - d0;
-        float dist = (d0 + d1) * 0.5f;
-        float h = norm(origin + ray * dist) - bottom;
-        od = od + extinction_of(k, density(k, h)) * step;
-    }
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) {
-        od.x += __shfl_xor(od.x, off, G);
-        od.y += __shfl_xor(od.y, off, G);
-        od.z += __shfl_xor(od.z, off, G);
-    }
-    if (active && lane == 0) st3(tlut, texel, clamp3(exp3(-mins(od, 80.0f)), 0.0f, 1.0f));
-}
-
-// _build_multi_scattering (renderer.py:1065-1206).  One wave64 per texel,
-// lane = Fibonacci-sphere direction.
-__device__ __forceinline__ void multi_scattering_body(const Consts& k, float bottom, f3 irradiance,
-                                                      float ar, const float* __restrict__ tlut,
-                                                      float* __restrict__ mslut, long gid) {
-    const long wave = gid / 64;
-    const int lane = threadIdx.x % 64;
-    const long n = (long)k.mw * k.mh;
-    if (wave >= n) return;  // uniform per wave
-    const int x = (int)(wave / k.mh), y = (int)(wave % k.mh);
-    float u = ((float)x + 0.5f) / (float)k.mw;
-    float v = ((float)y + 0.5f) / (float)k.mh;
-    float h0 = v * v * k.top;
-    float radius = bottom + h0;
-    float sun_c = sun_u_to_cosine(u, h0, bottom);
-    float sun_s = sqrtf(fmaxf(1.0f - sun_c * sun_c, 0.0f));
-    f3 sun = mk(sun_s, sun_c, 0.0f);
-    f3 origin = mk(0.0f, radius, 0.0f);
-    const int N = k.ms_dirs;
-    const int S = k.ms_steps;
-    f3 lum = mk(0.0f, 0.0f, 0.0f), fb = mk(0.0f, 0.0f, 0.0f);
-    for (int di = lane; di < N; di += 64) {
-        float si = (float)di + 0.5f;
-        float dc = 1.0f - 2.0f * si / (float)N;
-        float ds = sqrtf(fmaxf(1.0f - dc * dc, 0.0f));
-        float az = (float)di * 2.39996323f;
-        f3 ray = mk(ds * cr_cos(az), dc, ds * cr_sin(az));
-        float start, end;
-        bool valid, ground;
-        segment_vec(k, origin, ray, bottom, start, end, valid, ground);
-        f3 dl = mk(0.0f, 0.0f, 0.0f), df = mk(0.0f, 0.0f, 0.0f), pt = mk(1.0f, 1.0f, 1.0f);
-        if (valid) {
-            Warp w = make_warp(-dot(origin, ray), start, end);
-            RayPhase ph = ray_phase(k, ray, sun);
-            float d0 = boundary(w, 0.0f / (float)S);
-            for (int s = 0; s < S; ++s) {
-                float d1 = boundary(w, (float)(s + 1) / (float)S);
-                float dist = (d0 + d1) * 0.5f;
-                float step = d1 - d0;
-                f3 point = origin + ray * dist;
-                float h = altitude_on_ray(h0, dc, dist, bottom);
-                f3 d = density(k, h);
-                f3 st, si3;
-                segment_integral(extinction_of(k, d), step, st, si3);
-                f3 direct = scattering_source<false>(k, tlut, nullptr, point, h, d, ph, sun,
-                                                     irradiance, bottom, ar);
-                dl = dl + (pt * direct) * si3;
-                df = df + (pt * scattering_of(k, d)) * si3;
-                pt = pt * st;
-                d0 = d1;
-            }
-            if (ground) {
-                f3 gp = origin + ray * end;
-                f3 gn = normalized(gp);
-                float gsc = dot(gn, sun);
-                SunGeom g = sun_geometry(k, 0.0f, gsc, bottom, ar);
-                f3 gt = solar_transmittance(k, tlut, g, 0.0f, bottom, ar);
-                f3 gl = (((arr3(k.albedo) * irradiance) * gt) * fmaxf(gsc, 0.0f)) / k.pi;
-                dl = dl + pt * gl;
-            }
-        }
-        lum = lum + dl;
-        fb = fb + df;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        lum.x += __shfl_xor(lum.x, off, 64);
-        lum.y += __shfl_xor(lum.y, off, 64);
-        lum.z += __shfl_xor(lum.z, off, 64);
-        fb.x += __shfl_xor(fb.x, off, 64);
-        fb.y += __shfl_xor(fb.y, off, 64);
-        fb.z += __shfl_xor(fb.z, off, 64);
-    }
-    if (lane == 0) {
-        f3 avg_l = lum / (float)N;
-        f3 avg_f = clamp3(fb / (float)N, 0.0f, 0.95f);
-        st3(mslut, wave, maxs(avg_l / (mk(1.0f, 1.0f, 1.0f) - avg_f), 0.0f));
-    }
-}
-
-// _build_sky_view (renderer.py:1208-1302).  G lanes per texel, ordered
-// (L, T) scan over the lanes' step chunks.
-template <int G>
-__device__ __forceinline__ void sky_view_body(const Consts& k, float bottom, float cam_alt,
-                                              float sun_zc, f3 irradiance, float ar,
-                                              const float* __restrict__ tlut,
-                                              const float* __restrict__ mslut,
-                                              float* __restrict__ svlut, long gid) {
-    const int lane = threadIdx.x % G;
-    const long n = (long)k.sw * k.sh;
-    const long texel_raw = gid / G;
-    const bool active = texel_raw < n;
-    const long texel = active ? texel_raw : n - 1;
-    const int x = (int)(texel / k.sh), y = (int)(texel % k.sh);
-    float sun_s = sqrtf(fmaxf(1.0f - sun_zc * sun_zc, 0.0f));
-    f3 sun = mk(sun_s, sun_zc, 0.0f);
-    float safe_alt = fmaxf(cam_alt, 0.0f);
-    f3 origin = mk(0.0f, bottom + safe_alt, 0.0f);
-    float az = ((float)x + 0.5f) / (float)k.sw * k.pi;
-    float view_v = ((float)y + 0.5f) / (float)k.sh;
-    float zenith = sky_v_to_zenith(k, view_v, sky_horizon_zenith(k, safe_alt, bottom));
-    float vc = cr_cos(zenith);
-    float vsn = cr_sin(zenith);
-    f3 ray = mk(vsn * cr_cos(az), vc, vsn * cr_sin(az));
-    float start, end;
-    bool valid, ground;
-    segment_from_altitude(k, safe_alt, vc, bottom, start, end, valid, ground);
-    f3 L = mk(0.0f, 0.0f, 0.0f), T = mk(1.0f, 1.0f, 1.0f);
-    if (valid) {
-        Warp w = make_warp(-dot(origin, ray), start, end);
-        RayPhase ph = ray_phase(k, ray, sun);
-        const int steps = k.sv_steps;
-        const int per = (steps + G - 1) / G;
-        const int s0 = min(lane * per, steps), s1 = min(s0 + per, steps);
-        for (int s = s0; s < s1; ++s) {
-            float d0 = boundary(w, (float)s / (float)steps);
-            float d1 = boundary(w, (float)(s + 1) / (float)steps);
-            float dist = (d0 + d1) * 0.5f;
-            float step = d1 - d0;
-            f3 point = origin + ray * dist;
-            float h = altitude_on_ray(safe_alt, vc, dist, bottom);
-            f3 d = density(k, h);
-            f3 src = scattering_source<true>(k, tlut, mslut, point, h, d, ph, sun, irradiance,
-                                             bottom, ar);
-            f3 st, si3;
-            segment_integral(extinction_of(k, d), step, st, si3);
-            L = L + (T * src) * si3;
-            T = T * st;
-        }
-    }
-    scan_segments<G>(L, T, lane);
-    if (active && lane == G - 1) st3(svlut, texel, maxs(L, 0.0f));
-}
-
-// _build_aerial_perspective (renderer.py:1304-1420).  One wave64 per froxel
-// ray, lane = depth slice; slices beyond 64 are processed in chunks with the
-// previous chunk's composition carried in.
-__device__ __forceinline__ void aerial_body(const Consts& k, int width, int height, float bottom,
-                                            float cam_alt, f3 sun, f3 irradiance, float ar, f3 right,
-                                            f3 up, f3 forward, float thf,
-                                            const float* __restrict__ tlut,
-                                            const float* __restrict__ mslut,
-                                            float* __restrict__ ap, long gid) {
-    const long wave = gid / 64;
-    const int lane = threadIdx.x % 64;
-    const long n = (long)k.aw * k.ah;
-    if (wave >= n) return;  // uniform per wave
-    const int x = (int)(wave / k.ah), y = (int)(wave % k.ah);
-    float aspect = (float)width / (float)height;
-    f3 origin = mk(0.0f, bottom + cam_alt, 0.0f);
-    float u = ((float)x + 0.5f) / (float)k.aw;
-    float v = ((float)y + 0.5f) / (float)k.ah;
-    float sx = (u * 2.0f - 1.0f) * aspect;
-    float sy = v * 2.0f - 1.0f;
-    f3 ray = normalized((right * sx + up * sy) + forward / thf);
-    float a_start, a_end;
-    bool valid;
-    aerial_prefix(k, cam_alt, ray.y, bottom, a_start, a_end, valid);
-    RayPhase ph = ray_phase(k, ray, sun);
-    const float neg_od = -dot(origin, ray);
-    const int D = k.ad;
-    const int spp = k.ap_spp;
-    f3 carry_l = mk(0.0f, 0.0f, 0.0f), carry_t = mk(1.0f, 1.0f, 1.0f);
-    for (int base = 0; base < D; base += 64) {
-        const int di = base + lane;
-        f3 L = mk(0.0f, 0.0f, 0.0f), T = mk(1.0f, 1.0f, 1.0f);
-        if (valid && di < D) {
-            float previous = di == 0 ? a_start
-                                     : a_start + (a_end - a_start) * ((float)(di - 1) / (float)(D - 1));
-            float target = a_start + (a_end - a_start) * ((float)di / (float)(D - 1));
-            float is = fmaxf(previous, a_start);
-            float ie = fminf(target, a_end);
-            if (ie > is) {
-                Warp w = make_warp(neg_od, is, ie);
-                float d0 = boundary(w, 0.0f / (float)spp);
-                for (int s = 0; s < spp; ++s) {
-                    float d1 = boundary(w, (float)(s + 1) / (float)spp);
-                    float dist = (d0 + d1) * 0.5f;
-                    float step = d1 - d0;
-                    f3 point = origin + ray * dist;
-                    float h = altitude_on_ray(cam_alt, ray.y, dist, bottom);
-                    f3 d = density(k, h);
-                    f3 st, si3;
-                    segment_integral(extinction_of(k, d), step, st, si3);
-                    f3 src = scattering_source<true>(k, tlut, mslut, point, h, d, ph, sun,
-                                                     irradiance, bottom, ar);
-                    L = L + (T * src) * si3;
-                    T = T * st;
-                    d0 = d1;
-                }
-            }
-        }
-        scan_segments<64>(L, T, lane);
-        f3 cl = carry_l + carry_t * L;
-        f3 ct = carry_t * T;
-        if (di < D) st3(ap, wave * D + di, maxs(cl, 0.0f));
-        carry_l = mk(__shfl(cl.x, 63, 64), __shfl(cl.y, 63, 64), __shfl(cl.z, 63, 64));
-        carry_t = mk(__shfl(ct.x, 63, 64), __shfl(ct.y, 63, 64), __shfl(ct.z, 63, 64));
-    }
-}
-
-
-// Single-state kernels: one Taichi kernel call each.
-template <int G>
-__global__ void __launch_bounds__(256) k_transmittance(Consts k, float bottom, float* __restrict__ tlut) {
-    transmittance_body<G>(k, bottom, tlut, (long)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-__global__ void __launch_bounds__(256) k_multi_scattering(Consts k, float bottom, f3 irradiance,
-                                                          float ar, const float* __restrict__ tlut,
-                                                          float* __restrict__ mslut) {
-    multi_scattering_body(k, bottom, irradiance, ar, tlut, mslut,
-                          (long)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-template <int G>
-__global__ void __launch_bounds__(256) k_sky_view(Consts k, float bottom, float cam_alt, float sun_zc,
-                                                  f3 irradiance, float ar,
-                                                  const float* __restrict__ tlut,
-                                                  const float* __restrict__ mslut,
-                                                  float* __restrict__ svlut) {
-    sky_view_body<G>(k, bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut, svlut,
-                     (long)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-__global__ void __launch_bounds__(256) k_aerial(Consts k, int width, int height, float bottom,
-                                                float cam_alt, f3 sun, f3 irradiance, float ar,
-                                                f3 right, f3 up, f3 forward, float thf,
-                                                const float* __restrict__ tlut,
-                                                const float* __restrict__ mslut,
-                                                float* __restrict__ ap) {
-    aerial_body(k, width, height, bottom, cam_alt, sun, irradiance, ar, right, up, forward, thf, tlut,
-                mslut, ap, (long)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-// Batched kernels: blockIdx.y selects the state, whose float32 kernel
-// arguments and LUT slices come from the device state table; the same
-// bodies as the single-state kernels, so every state's LUTs are exactly
-// what four single-state calls would produce.
-template <int G>
-__global__ void __launch_bounds__(256) k_transmittance_batched(const BatchState* __restrict__ st,
-                                                               float* __restrict__ tluts,
-                                                               long stride) {
-    const BatchState& b = st[blockIdx.y];
-    transmittance_body<G>(b.k, b.bottom, tluts + blockIdx.y * stride,
-                          (long)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-__global__ void __launch_bounds__(256) k_multi_scattering_batched(const BatchState* __restrict__ st,
-                                                                  const float* __restrict__ tluts,
-                                                                  long t_stride,
-                                                                  float* __restrict__ msluts,
-                                                                  long m_stride) {
-    const BatchState& b = st[blockIdx.y];
-    multi_scattering_body(b.k, b.bottom, b.irradiance, b.ar_update, tluts + blockIdx.y * t_stride,
-                          msluts + blockIdx.y * m_stride, (long)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-template <int G>
-__global__ void __launch_bounds__(256) k_sky_view_batched(const BatchState* __restrict__ st,
-                                                          const float* __restrict__ tluts, long t_stride,
-                                                          const float* __restrict__ msluts,
-                                                          long m_stride, float* __restrict__ svluts,
-                                                          long s_stride) {
-    const BatchState& b = st[blockIdx.y];
-    sky_view_body<G>(b.k, b.bottom, b.sky_alt, b.sun_zc, b.irradiance, b.ar_update,
-                     tluts + blockIdx.y * t_stride, msluts + blockIdx.y * m_stride,
-                     svluts + blockIdx.y * s_stride, (long)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-__global__ void __launch_bounds__(256) k_aerial_batched(const BatchState* __restrict__ st, int width,
-                                                        int height, const float* __restrict__ tluts,
-                                                        long t_stride, const float* __restrict__ msluts,
-                                                        long m_stride, float* __restrict__ aps,
-                                                        long a_stride) {
-    const BatchState& b = st[blockIdx.y];
-    aerial_body(b.k, width, height, b.bottom, b.cam_alt, b.sun, b.irradiance, b.ar, b.right, b.up,
-                b.forward, b.thf, tluts + blockIdx.y * t_stride, msluts + blockIdx.y * m_stride,
-                aps + blockIdx.y * a_stride, (long)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-// _prepare_surface_lighting (renderer.py:1422-1472).  One thread per pixel.
-__global__ void __launch_bounds__(256) k_prepare(Consts k, long npix,
-                                                 const float* __restrict__ position_view,
-                                                 const int32_t* __restrict__ surface_id,
-                                                 float bottom, float cam_alt, f3 sun, float ar,
-                                                 f3 right, f3 up, f3 forward,
-                                                 const float* __restrict__ tlut,
-                                                 const float* __restrict__ mslut,
-                                                 float* __restrict__ sun_t,
-                                                 float* __restrict__ sky_l) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npix) return;
-    f3 solar = mk(1.0f, 1.0f, 1.0f);
-    f3 sky = mk(0.0f, 0.0f, 0.0f);
-    if (surface_id[i] >= 0) {
-        f3 camera = mk(0.0f, bottom + cam_alt, 0.0f);
-        f3 vp = ld3(position_view, i);
-        f3 local = (right * vp.x + up * vp.y) + forward * vp.z;
-        f3 planet = camera + local;
-        f3 radial = planet / fmaxf(norm(planet), 1.0f);
-        float sd = norm(local);
-        f3 local_ray = local / fmaxf(sd, 1.0f);
-        float h = altitude_on_ray(cam_alt, local_ray.y, sd, bottom);
-        float sc = dot(radial, sun);
-        SunGeom g = sun_geometry(k, h, sc, bottom, ar);
-        solar = solar_transmittance(k, tlut, g, h, bottom, ar);
-        sky = sample_multi_scattering(k, mslut, h, sc, bottom);
-    }
-    st3(sun_t, i, solar);
-    st3(sky_l, i, maxs(sky, 0.0f));
-}
-
-
-__device__ __forceinline__ f3 pixel_ray(const CompositeArgs& a, int px, int py, float& su,
-                                        float& sv) {
-    float aspect = (float)a.width / (float)a.height;
-    su = ((float)px + 0.5f) / (float)a.width;
-    sv = ((float)py + 0.5f) / (float)a.height;
-    float sx = (su * 2.0f - 1.0f) * aspect;
-    float sy = sv * 2.0f - 1.0f;
-    return normalized((a.right * sx + a.up * sy) + a.forward / a.thf);
-}
-
-// The COMPOSITE view of _composite (renderer.py:1860-1940) as two passes
-// over the same pixels: sky pixels (k_composite_sky) and surface pixels
-// (k_composite_surface).  Each pass walks 256-pixel groups with persistent
-// blocks and compacts its own pixel class onto the leading threads, so
-// waves run full and each pass gets the register budget of its own branch
-// (the surface branch alone fits three waves per SIMD).  Every pixel is
-// written by exactly one thread of one pass.
-__global__ void __launch_bounds__(256) k_composite_sky(Consts k, CompositeArgs a) {
-    __shared__ int list[256];
-    __shared__ int wave_counts[4];
-    const long npix = (long)a.width * a.height;
-    const long groups = (npix + 255) / 256;
-    for (long g = blockIdx.x; g < groups; g += gridDim.x) {
-        const long base = g * 256;
-        const long i = base + threadIdx.x;
-        const bool sky = i < npix && a.sid[i] < 0;
-        const int count = compact_block(sky, list, wave_counts);
-        if ((int)threadIdx.x < count) {
-            const long j = base + list[threadIdx.x];
-            f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height,
-                                a.right, a.up, a.forward, a.thf);
-            f3 c = composite_sky(k, a.tlut, a.svlut, ray, ld3(a.scene, j), ld3(a.space, j),
-                                 a.cam_alt, a.sky_alt, a.sun, a.disk, a.ar, a.thf, a.height, a.c0,
-                                 a.c1, a.bottom);
-            st3(a.hdr, j, maxs(c, 0.0f));
-        }
-        __syncthreads();  // `list` is rewritten by the next group
-    }
-}
-
-// Three waves per SIMD (at most 168 VGPRs, no spills) measured 15 % faster
-// than the compiler's unconstrained two-wave allocation at 1080p.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
-k_composite_surface(Consts k, CompositeArgs a) {
-    __shared__ int list[256];
-    __shared__ int wave_counts[4];
-    const long npix = (long)a.width * a.height;
-    const long groups = (npix + 255) / 256;
-    for (long g = blockIdx.x; g < groups; g += gridDim.x) {
-        const long base = g * 256;
-        const long i = base + threadIdx.x;
-        const bool surface = i < npix && a.sid[i] >= 0;
-        const int count = compact_block(surface, list, wave_counts);
-        if ((int)threadIdx.x < count) {
-            const long j = base + list[threadIdx.x];
-            f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height,
-                                a.right, a.up, a.forward, a.thf);
-            float sd = norm(ld3(a.pos, j));
-            f3 scat = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun,
-                                        a.irradiance, a.bottom, a.ar);
-            f3 trans = surface_transmittance(k, a.tlut, a.cam_alt, ray, sd, a.bottom);
-            st3(a.hdr, j, maxs(scat + ld3(a.scene, j) * trans, 0.0f));
-        }
-        __syncthreads();
-    }
-}
-
-// Diagnostic views 1..10 of _composite (renderer.py:1941-2055).
-__global__ void __launch_bounds__(256) k_composite_diagnostic(Consts k, CompositeArgs a) {
-    const long npix = (long)a.width * a.height;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npix) return;
-    const int px = (int)(i / a.height), py = (int)(i % a.height);
-    float su, sv;
-    f3 ray = pixel_ray(a, px, py, su, sv);
-    const bool surface = a.sid[i] >= 0;
-    f3 color = mk(0.0f, 0.0f, 0.0f);
-    switch (a.view) {
-        case ATMO_VIEW_SKY_VIEW:
-            color = sample_sky_view(k, a.svlut, ray, a.sun, a.sky_alt, a.bottom);
-            break;
-        case ATMO_VIEW_CAMERA_TRANSMITTANCE:
-            color = camera_transmittance(k, a.tlut, a.cam_alt, ray, a.bottom);
-            break;
-        case ATMO_VIEW_TRANSMITTANCE_LUT: {
-            int lx = min((int)(su * (float)k.tw), k.tw - 1);
-            int ly = min((int)(sv * (float)k.th), k.th - 1);
-            color = ld3(a.tlut, (long)lx * k.th + ly);
-            break;
-        }
-        case ATMO_VIEW_MULTI_SCATTERING_LUT: {
-            int lx = min((int)(su * (float)k.mw), k.mw - 1);
-            int ly = min((int)(sv * (float)k.mh), k.mh - 1);
-            color = ld3(a.mslut, (long)lx * k.mh + ly);
-            break;
-        }
-        case ATMO_VIEW_AERIAL_SCATTERING:
-        case ATMO_VIEW_AERIAL_DIRECT_ONLY:
-            if (surface)
-                color = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, norm(ld3(a.pos, i)),
-                                          a.sun, a.irradiance, a.bottom, a.ar);
-            break;
-        case ATMO_VIEW_AERIAL_FROXEL_ONLY:
-            if (surface)
-                color = sample_aerial(k, a.aplut, su, sv, norm(ld3(a.pos, i)), a.cam_alt, ray,
-                                      a.bottom);
-            break;
-        case ATMO_VIEW_AERIAL_BLEND_WEIGHT: {
-            float w = surface ? 1.0f : 0.0f;
-            color = mk(w, w, w);
-            break;
-        }
-        case ATMO_VIEW_AERIAL_TRANSMITTANCE:
-            if (surface)
-                color = surface_transmittance(k, a.tlut, a.cam_alt, ray, norm(ld3(a.pos, i)),
-                                              a.bottom);
-            break;
-        default: {
-            float m = surface ? 1.0f : 0.0f;
-            color = mk(m, m, m);
-            break;
-        }
-    }
-    st3(a.hdr, i, maxs(color, 0.0f));
-}
 
 // ------------------------------------------------------------ C-ABI
 
 namespace {
 
-constexpr int kGroupT = 8;   // lanes per transmittance texel
-constexpr int kGroupSV = 8;  // lanes per sky-view texel
-
 unsigned blocks_for(long threads, int block) { return (unsigned)((threads + block - 1) / block); }
-
-// Persistent grid of the composite passes: enough 256-thread blocks for
-// several per CU on 256 CUs, never more than there are 256-pixel groups.
-unsigned composite_blocks(long npix) {
-    const long groups = (npix + 255) / 256;
-    return (unsigned)(groups < 2048 ? groups : 2048);
-}
 
 float** buffer_slot(AtmoHandle* h, int which) {
     switch (which) {
@@ -835,11 +325,20 @@ int32_t atmo_bind_buffer(AtmoHandle* h, int32_t which, void* dptr, int64_t bytes
     return ATMO_OK;
 }
 
+int32_t atmo_set_precision(AtmoHandle* h, int32_t precision) {
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
+    if (precision != ATMO_PRECISION_REFERENCE && precision != ATMO_PRECISION_ROBUST) {
+        set_error(h, "atmo_set_precision: precision must be ATMO_PRECISION_REFERENCE or "
+                     "ATMO_PRECISION_ROBUST");
+        return ATMO_EINVAL;
+    }
+    h->precision = precision;
+    return ATMO_OK;
+}
+
 int32_t atmo_build_transmittance(AtmoHandle* h, float bottom, void* stream) {
     if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
-    const long threads = (long)h->k.tw * h->k.th * kGroupT;
-    hipLaunchKernelGGL(k_transmittance<kGroupT>, dim3(blocks_for(threads, 256)), dim3(256), 0,
-                       (hipStream_t)stream, h->k, bottom, h->tlut);
+    ATMO_STAGE(h, launch_transmittance, h->k, bottom, h->tlut, (hipStream_t)stream);
     return atmo_launch_status(h, "atmo_build_transmittance");
 }
 
@@ -850,10 +349,8 @@ int32_t atmo_build_multi_scattering(AtmoHandle* h, float bottom, const float irr
         set_error(h, "atmo_build_multi_scattering: NULL irradiance");
         return ATMO_EINVAL;
     }
-    const long threads = (long)h->k.mw * h->k.mh * 64;
-    hipLaunchKernelGGL(k_multi_scattering, dim3(blocks_for(threads, 256)), dim3(256), 0,
-                       (hipStream_t)stream, h->k, bottom, host3(irradiance), ar, h->tlut,
-                       h->mslut);
+    ATMO_STAGE(h, launch_multi_scattering, h->k, bottom, host3(irradiance), ar, h->tlut, h->mslut,
+               (hipStream_t)stream);
     return atmo_launch_status(h, "atmo_build_multi_scattering");
 }
 
@@ -864,10 +361,8 @@ int32_t atmo_build_sky_view(AtmoHandle* h, float bottom, float cam_alt, float su
         set_error(h, "atmo_build_sky_view: NULL irradiance");
         return ATMO_EINVAL;
     }
-    const long threads = (long)h->k.sw * h->k.sh * kGroupSV;
-    hipLaunchKernelGGL(k_sky_view<kGroupSV>, dim3(blocks_for(threads, 256)), dim3(256), 0,
-                       (hipStream_t)stream, h->k, bottom, cam_alt, sun_zc, host3(irradiance), ar,
-                       h->tlut, h->mslut, h->svlut);
+    ATMO_STAGE(h, launch_sky_view, h->k, bottom, cam_alt, sun_zc, host3(irradiance), ar, h->tlut,
+               h->mslut, h->svlut, (hipStream_t)stream);
     return atmo_launch_status(h, "atmo_build_sky_view");
 }
 
@@ -880,11 +375,9 @@ int32_t atmo_build_aerial_perspective(AtmoHandle* h, float bottom, float cam_alt
         set_error(h, "atmo_build_aerial_perspective: NULL vector argument");
         return ATMO_EINVAL;
     }
-    const long threads = (long)h->k.aw * h->k.ah * 64;
-    hipLaunchKernelGGL(k_aerial, dim3(blocks_for(threads, 256)), dim3(256), 0,
-                       (hipStream_t)stream, h->k, h->width, h->height, bottom, cam_alt,
-                       host3(sun), host3(irradiance), ar, host3(right), host3(up),
-                       host3(forward), thf, h->tlut, h->mslut, h->aplut);
+    ATMO_STAGE(h, launch_aerial, h->k, h->width, h->height, bottom, cam_alt, host3(sun),
+               host3(irradiance), ar, host3(right), host3(up), host3(forward), thf, h->tlut,
+               h->mslut, h->aplut, (hipStream_t)stream);
     return atmo_launch_status(h, "atmo_build_aerial_perspective");
 }
 
@@ -898,19 +391,23 @@ int32_t atmo_prepare_surface_lighting(AtmoHandle* h, const float* position_view,
         return ATMO_EINVAL;
     }
     const long npix = (long)h->width * h->height;
-    hipLaunchKernelGGL(k_prepare, dim3(blocks_for(npix, 256)), dim3(256), 0, (hipStream_t)stream,
-                       h->k, npix, position_view, surface_id, bottom, cam_alt, host3(sun), ar,
-                       host3(right), host3(up), host3(forward), h->tlut, h->mslut, h->sun_t,
-                       h->sky_l);
+    ATMO_STAGE(h, launch_prepare, h->k, npix, position_view, surface_id, bottom, cam_alt,
+               host3(sun), ar, host3(right), host3(up), host3(forward), h->tlut, h->mslut,
+               h->sun_t, h->sky_l, (hipStream_t)stream);
     return atmo_launch_status(h, "atmo_prepare_surface_lighting");
 }
 
 }  // extern "C"
 
 int32_t atmo_launch_composite_sky(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream) {
-    const long npix = (long)a.width * a.height;
-    hipLaunchKernelGGL(k_composite_sky, dim3(composite_blocks(npix)), dim3(256), 0, stream, h->k, a);
+    ATMO_STAGE(h, launch_composite_sky, h->k, a, stream);
     return atmo_launch_status(h, "k_composite_sky");
+}
+
+int32_t atmo_launch_composite_list(AtmoHandle* h, const CompositeArgs& a, const int* list,
+                                   hipStream_t stream) {
+    ATMO_STAGE(h, launch_composite_list, h->k, a, list, stream);
+    return atmo_launch_status(h, "k_composite_list");
 }
 
 extern "C" {
@@ -957,15 +454,7 @@ int32_t atmo_composite(AtmoHandle* h, const float* scene_hdr, const float* space
     a.c0 = c0;
     a.c1 = c1;
     a.view = view;
-    const long npix = (long)h->width * h->height;
-    if (view == ATMO_VIEW_COMPOSITE) {
-        const unsigned blocks = composite_blocks(npix);
-        hipLaunchKernelGGL(k_composite_sky, dim3(blocks), dim3(256), 0, (hipStream_t)stream, h->k, a);
-        hipLaunchKernelGGL(k_composite_surface, dim3(blocks), dim3(256), 0, (hipStream_t)stream, h->k,
-                           a);
-    } else
-        hipLaunchKernelGGL(k_composite_diagnostic, dim3(blocks_for(npix, 256)), dim3(256), 0,
-                           (hipStream_t)stream, h->k, a);
+    ATMO_STAGE(h, launch_composite, h->k, a, (hipStream_t)stream);
     return atmo_launch_status(h, "atmo_composite");
 }
 
@@ -1140,20 +629,7 @@ int32_t atmo_batch_run(AtmoBatch* b, void* stream) {
             return ATMO_EHIP;
         }
         hipStream_t cs = b->capture;
-        hipLaunchKernelGGL(k_transmittance_batched<kGroupT>,
-                           dim3(blocks_for((long)k.tw * k.th * kGroupT, 256), n), dim3(256), 0, cs,
-                           b->states, b->luts[0], b->strides[0]);
-        hipLaunchKernelGGL(k_multi_scattering_batched, dim3(blocks_for((long)k.mw * k.mh * 64, 256), n),
-                           dim3(256), 0, cs, b->states, (const float*)b->luts[0], b->strides[0],
-                           b->luts[1], b->strides[1]);
-        hipLaunchKernelGGL(k_sky_view_batched<kGroupSV>,
-                           dim3(blocks_for((long)k.sw * k.sh * kGroupSV, 256), n), dim3(256), 0, cs,
-                           b->states, (const float*)b->luts[0], b->strides[0],
-                           (const float*)b->luts[1], b->strides[1], b->luts[2], b->strides[2]);
-        hipLaunchKernelGGL(k_aerial_batched, dim3(blocks_for((long)k.aw * k.ah * 64, 256), n), dim3(256),
-                           0, cs, b->states, b->width, b->height, (const float*)b->luts[0],
-                           b->strides[0], (const float*)b->luts[1], b->strides[1], b->luts[3],
-                           b->strides[3]);
+        atmo::ref::launch_batch(k, b->states, n, b->width, b->height, b->luts, b->strides, cs);
         hipError_t launch = hipGetLastError();
         hipError_t end = hipStreamEndCapture(cs, &b->graph);
         if (launch != hipSuccess || end != hipSuccess ||

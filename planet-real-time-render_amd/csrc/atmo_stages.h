@@ -1,0 +1,692 @@
+// atmo_stages.h -- the atmosphere kernels and their launchers, compiled
+// once per precision variant (see atmo_device.h): atmo_kernels.hip includes
+// it for atmo::ref, atmo_robust.hip (ATMO_ROBUST 1) for atmo::robust.
+//
+// One kernel per Taichi kernel of the reference's AtmosphereRenderer
+// (src/planet_renderer/atmosphere/renderer.py:1034-2055).  Launch shapes:
+//  - transmittance / sky-view: a group of G lanes per texel; each lane
+//    marches a contiguous chunk of the texel's steps and the group combines
+//    the chunks (a sum of optical depth for T, an ordered (L, T) scan for
+//    sky-view).  A single-state LUT has only 16-20 K texels, so one thread
+//    per texel would leave most of the 256 CUs idle and expose the full
+//    serial march latency.
+//  - multi-scattering: one wave64 per texel, lane = sphere direction, the
+//    64-direction integral reduced with cross-lane shuffles.
+//  - aerial perspective: one wave64 per froxel ray, lane = depth slice; the
+//    cumulative in-scatter along the ray is an ordered (L, T) scan.
+//  - prepare / composite: one thread per pixel in the (W, H) field order,
+//    so consecutive lanes read and write consecutive 12-byte elements.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/atmo.h"
+#include "atmo_device.h"
+#include "atmo_handle.h"
+
+namespace atmo {
+inline namespace ATMO_PRECISION {
+
+// Ordered combination of consecutive (L, T) segments within a lane group of
+// width G: afterwards lane i holds the composition of segments 0..i
+// (earlier segment first: L = L' + T' * L, T = T' * T).
+template <int G>
+__device__ __forceinline__ void scan_segments(f3& L, f3& T, int lane) {
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+        f3 pl = mk(__shfl_up(L.x, off, G), __shfl_up(L.y, off, G), __shfl_up(L.z, off, G));
+        f3 pt = mk(__shfl_up(T.x, off, G), __shfl_up(T.y, off, G), __shfl_up(T.z, off, G));
+        if (lane >= off) {
+            L = pl + pt * L;
+            T = pt * T;
+        }
+    }
+}
+
+// ------------------------------------------------------------ kernels
+
+// _build_transmittance (renderer.py:1034-1063) with _integrate_extinction
+// (renderer.py:570-607).  G lanes per texel.
+template <int G>
+__device__ __forceinline__ void transmittance_body(const Consts& k, float bottom,
+                                                   float* __restrict__ tlut, long gid) {
+    const int lane = threadIdx.x % G;
+    const long n = (long)k.tw * k.th;
+    const long texel_raw = gid / G;
+    const bool active = texel_raw < n;
+    const long texel = active ? texel_raw : n - 1;
+    const int x = (int)(texel / k.th), y = (int)(texel % k.th);
+    float u = ((float)x + 0.5f) / (float)k.tw;
+    float v = ((float)y + 0.5f) / (float)k.th;
+    float alt, c, distance;
+    transmittance_uv_to_ray(k, u, v, bottom, alt, c, distance);
+    float sine = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+    f3 origin = mk(0.0f, bottom + alt, 0.0f);
+    f3 ray = mk(sine, c, 0.0f);
+    Warp w = make_warp(-dot(origin, ray), 0.0f, distance);
+    const int steps = k.t_steps;
+    const int per = (steps + G - 1) / G;
+    const int s0 = min(lane * per, steps), s1 = min(s0 + per, steps);
+    f3 od = mk(0.0f, 0.0f, 0.0f);
+#if ATMO_ROBUST
+    f3 carry = mk(0.0f, 0.0f, 0.0f);  // Kahan compensation of the lane's chunk
+#endif
+    for (int s = s0; s < s1; ++s) {
+        float d0 = boundary(w, (float)s / (float)steps);
+        float d1 = boundary(w, (float)(s + 1) / (float)steps);
+        float step = d1 - d0;
+        float dist = (d0 + d1) * 0.5f;
+#if ATMO_ROBUST
+        // |origin + ray d| - R cancels at planet scale (and the float32
+        // origin radius is itself rounded): the float64 altitude on the ray
+        float h = altitude_on_ray(alt, c, dist, bottom);
+        f3 y = extinction_of(k, density(k, h)) * step - carry;
+        f3 t = od + y;
+        carry = (t - od) - y;
+        od = t;
+#else
+        float h = norm(origin + ray * dist) - bottom;
+        od = od + extinction_of(k, density(k, h)) * step;
+#endif
+    }
+#if ATMO_ROBUST
+    // compensated tree over the lanes: each level adds the partner's sum
+    // with its rounding error (TwoSum) folded into the carried compensation
+    od = od - carry;
+    carry = mk(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        f3 o = mk(__shfl_xor(od.x, off, G), __shfl_xor(od.y, off, G), __shfl_xor(od.z, off, G));
+        f3 oc = mk(__shfl_xor(carry.x, off, G), __shfl_xor(carry.y, off, G),
+                   __shfl_xor(carry.z, off, G));
+        f3 t = od + o;
+        f3 bp = t - od;
+        f3 err = (od - (t - bp)) + (o - bp);
+        carry = (carry + oc) + err;
+        od = t;
+    }
+    od = od + carry;
+#else
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        od.x += __shfl_xor(od.x, off, G);
+        od.y += __shfl_xor(od.y, off, G);
+        od.z += __shfl_xor(od.z, off, G);
+    }
+#endif
+    if (active && lane == 0) st3(tlut, texel, clamp3(exp3(-mins(od, 80.0f)), 0.0f, 1.0f));
+}
+
+// _build_multi_scattering (renderer.py:1065-1206).  One wave64 per texel,
+// lane = Fibonacci-sphere direction.
+__device__ __forceinline__ void multi_scattering_body(const Consts& k, float bottom, f3 irradiance,
+                                                      float ar, const float* __restrict__ tlut,
+                                                      float* __restrict__ mslut, long gid) {
+    const long wave = gid / 64;
+    const int lane = threadIdx.x % 64;
+    const long n = (long)k.mw * k.mh;
+    if (wave >= n) return;  // uniform per wave
+    const int x = (int)(wave / k.mh), y = (int)(wave % k.mh);
+    float u = ((float)x + 0.5f) / (float)k.mw;
+    float v = ((float)y + 0.5f) / (float)k.mh;
+    float h0 = v * v * k.top;
+    float radius = bottom + h0;
+    float sun_c = sun_u_to_cosine(u, h0, bottom);
+    float sun_s = sqrtf(fmaxf(1.0f - sun_c * sun_c, 0.0f));
+    f3 sun = mk(sun_s, sun_c, 0.0f);
+    f3 origin = mk(0.0f, radius, 0.0f);
+    const int N = k.ms_dirs;
+    const int S = k.ms_steps;
+    f3 lum = mk(0.0f, 0.0f, 0.0f), fb = mk(0.0f, 0.0f, 0.0f);
+    for (int di = lane; di < N; di += 64) {
+        float si = (float)di + 0.5f;
+        float dc = 1.0f - 2.0f * si / (float)N;
+        float ds = sqrtf(fmaxf(1.0f - dc * dc, 0.0f));
+        float az = (float)di * 2.39996323f;
+        f3 ray = mk(ds * cr_cos(az), dc, ds * cr_sin(az));
+        float start, end;
+        bool valid, ground;
+        segment_vec(k, origin, ray, bottom, start, end, valid, ground);
+        f3 dl = mk(0.0f, 0.0f, 0.0f), df = mk(0.0f, 0.0f, 0.0f), pt = mk(1.0f, 1.0f, 1.0f);
+        if (valid) {
+            Warp w = make_warp(-dot(origin, ray), start, end);
+            RayPhase ph = ray_phase(k, ray, sun);
+            float d0 = boundary(w, 0.0f / (float)S);
+            for (int s = 0; s < S; ++s) {
+                float d1 = boundary(w, (float)(s + 1) / (float)S);
+                float dist = (d0 + d1) * 0.5f;
+                float step = d1 - d0;
+                f3 point = origin + ray * dist;
+                float h = altitude_on_ray(h0, dc, dist, bottom);
+                f3 d = density(k, h);
+                f3 st, si3;
+                segment_integral(extinction_of(k, d), step, st, si3);
+                f3 direct = scattering_source<false>(k, tlut, nullptr, point, h, d, ph, sun,
+                                                     irradiance, bottom, ar);
+                dl = dl + (pt * direct) * si3;
+                df = df + (pt * scattering_of(k, d)) * si3;
+                pt = pt * st;
+                d0 = d1;
+            }
+            if (ground) {
+                f3 gp = origin + ray * end;
+                f3 gn = normalized(gp);
+                float gsc = dot(gn, sun);
+                SunGeom g = sun_geometry(k, 0.0f, gsc, bottom, ar);
+                f3 gt = solar_transmittance(k, tlut, g, 0.0f, bottom, ar);
+                f3 gl = (((arr3(k.albedo) * irradiance) * gt) * fmaxf(gsc, 0.0f)) / k.pi;
+                dl = dl + pt * gl;
+            }
+        }
+        lum = lum + dl;
+        fb = fb + df;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lum.x += __shfl_xor(lum.x, off, 64);
+        lum.y += __shfl_xor(lum.y, off, 64);
+        lum.z += __shfl_xor(lum.z, off, 64);
+        fb.x += __shfl_xor(fb.x, off, 64);
+        fb.y += __shfl_xor(fb.y, off, 64);
+        fb.z += __shfl_xor(fb.z, off, 64);
+    }
+    if (lane == 0) {
+        f3 avg_l = lum / (float)N;
+        f3 avg_f = clamp3(fb / (float)N, 0.0f, 0.95f);
+        st3(mslut, wave, maxs(avg_l / (mk(1.0f, 1.0f, 1.0f) - avg_f), 0.0f));
+    }
+}
+
+// _build_sky_view (renderer.py:1208-1302).  G lanes per texel, ordered
+// (L, T) scan over the lanes' step chunks.
+template <int G>
+__device__ __forceinline__ void sky_view_body(const Consts& k, float bottom, float cam_alt,
+                                              float sun_zc, f3 irradiance, float ar,
+                                              const float* __restrict__ tlut,
+                                              const float* __restrict__ mslut,
+                                              float* __restrict__ svlut, long gid) {
+    const int lane = threadIdx.x % G;
+    const long n = (long)k.sw * k.sh;
+    const long texel_raw = gid / G;
+    const bool active = texel_raw < n;
+    const long texel = active ? texel_raw : n - 1;
+    const int x = (int)(texel / k.sh), y = (int)(texel % k.sh);
+    float sun_s = sqrtf(fmaxf(1.0f - sun_zc * sun_zc, 0.0f));
+    f3 sun = mk(sun_s, sun_zc, 0.0f);
+    float safe_alt = fmaxf(cam_alt, 0.0f);
+    f3 origin = mk(0.0f, bottom + safe_alt, 0.0f);
+    float az = ((float)x + 0.5f) / (float)k.sw * k.pi;
+    float view_v = ((float)y + 0.5f) / (float)k.sh;
+#if ATMO_ROBUST
+    const double zenith_w = sky_v_to_zenith_w(view_v, safe_alt, bottom);
+    float vc = (float)cos(zenith_w);
+    float vsn = (float)sin(zenith_w);
+#else
+    float zenith = sky_v_to_zenith(k, view_v, sky_horizon_zenith(k, safe_alt, bottom));
+    float vc = cr_cos(zenith);
+    float vsn = cr_sin(zenith);
+#endif
+    f3 ray = mk(vsn * cr_cos(az), vc, vsn * cr_sin(az));
+    float start, end;
+    bool valid, ground;
+    segment_from_altitude(k, safe_alt, vc, bottom, start, end, valid, ground);
+    f3 L = mk(0.0f, 0.0f, 0.0f), T = mk(1.0f, 1.0f, 1.0f);
+    if (valid) {
+        Warp w = make_warp(-dot(origin, ray), start, end);
+        RayPhase ph = ray_phase(k, ray, sun);
+        const int steps = k.sv_steps;
+        const int per = (steps + G - 1) / G;
+        const int s0 = min(lane * per, steps), s1 = min(s0 + per, steps);
+        for (int s = s0; s < s1; ++s) {
+            float d0 = boundary(w, (float)s / (float)steps);
+            float d1 = boundary(w, (float)(s + 1) / (float)steps);
+            float dist = (d0 + d1) * 0.5f;
+            float step = d1 - d0;
+            f3 point = origin + ray * dist;
+            float h = altitude_on_ray(safe_alt, vc, dist, bottom);
+            f3 d = density(k, h);
+            f3 src = scattering_source<true>(k, tlut, mslut, point, h, d, ph, sun, irradiance,
+                                             bottom, ar);
+            f3 st, si3;
+            segment_integral(extinction_of(k, d), step, st, si3);
+            L = L + (T * src) * si3;
+            T = T * st;
+        }
+    }
+    scan_segments<G>(L, T, lane);
+    if (active && lane == G - 1) st3(svlut, texel, maxs(L, 0.0f));
+}
+
+// _build_aerial_perspective (renderer.py:1304-1420).  One wave64 per froxel
+// ray, lane = depth slice; slices beyond 64 are processed in chunks with the
+// previous chunk's composition carried in.
+__device__ __forceinline__ void aerial_body(const Consts& k, int width, int height, float bottom,
+                                            float cam_alt, f3 sun, f3 irradiance, float ar, f3 right,
+                                            f3 up, f3 forward, float thf,
+                                            const float* __restrict__ tlut,
+                                            const float* __restrict__ mslut,
+                                            float* __restrict__ ap, long gid) {
+    const long wave = gid / 64;
+    const int lane = threadIdx.x % 64;
+    const long n = (long)k.aw * k.ah;
+    if (wave >= n) return;  // uniform per wave
+    const int x = (int)(wave / k.ah), y = (int)(wave % k.ah);
+    float aspect = (float)width / (float)height;
+    f3 origin = mk(0.0f, bottom + cam_alt, 0.0f);
+    float u = ((float)x + 0.5f) / (float)k.aw;
+    float v = ((float)y + 0.5f) / (float)k.ah;
+    float sx = (u * 2.0f - 1.0f) * aspect;
+    float sy = v * 2.0f - 1.0f;
+    f3 ray = normalized((right * sx + up * sy) + forward / thf);
+    float a_start, a_end;
+    bool valid;
+    aerial_prefix(k, cam_alt, ray.y, bottom, a_start, a_end, valid);
+    RayPhase ph = ray_phase(k, ray, sun);
+    const float neg_od = -dot(origin, ray);
+    const int D = k.ad;
+    const int spp = k.ap_spp;
+    f3 carry_l = mk(0.0f, 0.0f, 0.0f), carry_t = mk(1.0f, 1.0f, 1.0f);
+    for (int base = 0; base < D; base += 64) {
+        const int di = base + lane;
+        f3 L = mk(0.0f, 0.0f, 0.0f), T = mk(1.0f, 1.0f, 1.0f);
+        if (valid && di < D) {
+            float previous = di == 0 ? a_start
+                                     : a_start + (a_end - a_start) * ((float)(di - 1) / (float)(D - 1));
+            float target = a_start + (a_end - a_start) * ((float)di / (float)(D - 1));
+            float is = fmaxf(previous, a_start);
+            float ie = fminf(target, a_end);
+            if (ie > is) {
+                Warp w = make_warp(neg_od, is, ie);
+                float d0 = boundary(w, 0.0f / (float)spp);
+                for (int s = 0; s < spp; ++s) {
+                    float d1 = boundary(w, (float)(s + 1) / (float)spp);
+                    float dist = (d0 + d1) * 0.5f;
+                    float step = d1 - d0;
+                    f3 point = origin + ray * dist;
+                    float h = altitude_on_ray(cam_alt, ray.y, dist, bottom);
+                    f3 d = density(k, h);
+                    f3 st, si3;
+                    segment_integral(extinction_of(k, d), step, st, si3);
+                    f3 src = scattering_source<true>(k, tlut, mslut, point, h, d, ph, sun,
+                                                     irradiance, bottom, ar);
+                    L = L + (T * src) * si3;
+                    T = T * st;
+                    d0 = d1;
+                }
+            }
+        }
+        scan_segments<64>(L, T, lane);
+        f3 cl = carry_l + carry_t * L;
+        f3 ct = carry_t * T;
+        if (di < D) st3(ap, wave * D + di, maxs(cl, 0.0f));
+        carry_l = mk(__shfl(cl.x, 63, 64), __shfl(cl.y, 63, 64), __shfl(cl.z, 63, 64));
+        carry_t = mk(__shfl(ct.x, 63, 64), __shfl(ct.y, 63, 64), __shfl(ct.z, 63, 64));
+    }
+}
+
+
+// Single-state kernels: one Taichi kernel call each.
+template <int G>
+__global__ void __launch_bounds__(256) k_transmittance(Consts k, float bottom, float* __restrict__ tlut) {
+    transmittance_body<G>(k, bottom, tlut, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_multi_scattering(Consts k, float bottom, f3 irradiance,
+                                                          float ar, const float* __restrict__ tlut,
+                                                          float* __restrict__ mslut) {
+    multi_scattering_body(k, bottom, irradiance, ar, tlut, mslut,
+                          (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <int G>
+__global__ void __launch_bounds__(256) k_sky_view(Consts k, float bottom, float cam_alt, float sun_zc,
+                                                  f3 irradiance, float ar,
+                                                  const float* __restrict__ tlut,
+                                                  const float* __restrict__ mslut,
+                                                  float* __restrict__ svlut) {
+    sky_view_body<G>(k, bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut, svlut,
+                     (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_aerial(Consts k, int width, int height, float bottom,
+                                                float cam_alt, f3 sun, f3 irradiance, float ar,
+                                                f3 right, f3 up, f3 forward, float thf,
+                                                const float* __restrict__ tlut,
+                                                const float* __restrict__ mslut,
+                                                float* __restrict__ ap) {
+    aerial_body(k, width, height, bottom, cam_alt, sun, irradiance, ar, right, up, forward, thf, tlut,
+                mslut, ap, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+#if !ATMO_ROBUST
+// Batched kernels: blockIdx.y selects the state, whose float32 kernel
+// arguments and LUT slices come from the device state table; the same
+// bodies as the single-state kernels, so every state's LUTs are exactly
+// what four single-state calls would produce.
+template <int G>
+__global__ void __launch_bounds__(256) k_transmittance_batched(const BatchState* __restrict__ st,
+                                                               float* __restrict__ tluts,
+                                                               long stride) {
+    const BatchState& b = st[blockIdx.y];
+    transmittance_body<G>(b.k, b.bottom, tluts + blockIdx.y * stride,
+                          (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_multi_scattering_batched(const BatchState* __restrict__ st,
+                                                                  const float* __restrict__ tluts,
+                                                                  long t_stride,
+                                                                  float* __restrict__ msluts,
+                                                                  long m_stride) {
+    const BatchState& b = st[blockIdx.y];
+    multi_scattering_body(b.k, b.bottom, b.irradiance, b.ar_update, tluts + blockIdx.y * t_stride,
+                          msluts + blockIdx.y * m_stride, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <int G>
+__global__ void __launch_bounds__(256) k_sky_view_batched(const BatchState* __restrict__ st,
+                                                          const float* __restrict__ tluts, long t_stride,
+                                                          const float* __restrict__ msluts,
+                                                          long m_stride, float* __restrict__ svluts,
+                                                          long s_stride) {
+    const BatchState& b = st[blockIdx.y];
+    sky_view_body<G>(b.k, b.bottom, b.sky_alt, b.sun_zc, b.irradiance, b.ar_update,
+                     tluts + blockIdx.y * t_stride, msluts + blockIdx.y * m_stride,
+                     svluts + blockIdx.y * s_stride, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_aerial_batched(const BatchState* __restrict__ st, int width,
+                                                        int height, const float* __restrict__ tluts,
+                                                        long t_stride, const float* __restrict__ msluts,
+                                                        long m_stride, float* __restrict__ aps,
+                                                        long a_stride) {
+    const BatchState& b = st[blockIdx.y];
+    aerial_body(b.k, width, height, b.bottom, b.cam_alt, b.sun, b.irradiance, b.ar, b.right, b.up,
+                b.forward, b.thf, tluts + blockIdx.y * t_stride, msluts + blockIdx.y * m_stride,
+                aps + blockIdx.y * a_stride, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+#endif  // !ATMO_ROBUST
+
+// _prepare_surface_lighting (renderer.py:1422-1472).  One thread per pixel.
+__global__ void __launch_bounds__(256) k_prepare(Consts k, long npix,
+                                                 const float* __restrict__ position_view,
+                                                 const int32_t* __restrict__ surface_id,
+                                                 float bottom, float cam_alt, f3 sun, float ar,
+                                                 f3 right, f3 up, f3 forward,
+                                                 const float* __restrict__ tlut,
+                                                 const float* __restrict__ mslut,
+                                                 float* __restrict__ sun_t,
+                                                 float* __restrict__ sky_l) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    f3 solar = mk(1.0f, 1.0f, 1.0f);
+    f3 sky = mk(0.0f, 0.0f, 0.0f);
+    if (surface_id[i] >= 0) {
+        f3 camera = mk(0.0f, bottom + cam_alt, 0.0f);
+        f3 vp = ld3(position_view, i);
+        f3 local = (right * vp.x + up * vp.y) + forward * vp.z;
+        f3 planet = camera + local;
+        f3 radial = planet / fmaxf(norm(planet), 1.0f);
+        float sd = norm(local);
+        f3 local_ray = local / fmaxf(sd, 1.0f);
+        float sc = dot(radial, sun);
+#if ATMO_ROBUST
+        if (cam_alt > 1.0e6f) {
+            // float64 island: camera-relative endpoint above 1000 km, where
+            // the float32 camera radius has a metre-sized ulp
+            const double lx = ((double)right.x * vp.x + (double)up.x * vp.y) + (double)forward.x * vp.z;
+            const double ly = ((double)right.y * vp.x + (double)up.y * vp.y) + (double)forward.y * vp.z;
+            const double lz = ((double)right.z * vp.x + (double)up.z * vp.y) + (double)forward.z * vp.z;
+            const double py = ((double)bottom + cam_alt) + ly;
+            const double pn = fmax(sqrt((lx * lx + py * py) + lz * lz), 1.0);
+            sc = (float)(((lx * sun.x + py * sun.y) + lz * sun.z) / pn);
+            const double ld = sqrt((lx * lx + ly * ly) + lz * lz);
+            sd = (float)ld;
+            local_ray.y = (float)(ly / fmax(ld, 1.0));
+        }
+#endif
+        float h = altitude_on_ray(cam_alt, local_ray.y, sd, bottom);
+        SunGeom g = sun_geometry(k, h, sc, bottom, ar);
+        solar = solar_transmittance(k, tlut, g, h, bottom, ar);
+        sky = sample_multi_scattering(k, mslut, h, sc, bottom);
+    }
+    st3(sun_t, i, solar);
+    st3(sky_l, i, maxs(sky, 0.0f));
+}
+
+
+__device__ __forceinline__ f3 pixel_ray(const CompositeArgs& a, int px, int py, float& su,
+                                        float& sv) {
+    float aspect = (float)a.width / (float)a.height;
+    su = ((float)px + 0.5f) / (float)a.width;
+    sv = ((float)py + 0.5f) / (float)a.height;
+    float sx = (su * 2.0f - 1.0f) * aspect;
+    float sy = sv * 2.0f - 1.0f;
+    return normalized((a.right * sx + a.up * sy) + a.forward / a.thf);
+}
+
+// The COMPOSITE view of _composite (renderer.py:1860-1940) as two passes
+// over the same pixels: sky pixels (k_composite_sky) and surface pixels
+// (k_composite_surface).  Each pass walks 256-pixel groups with persistent
+// blocks and compacts its own pixel class onto the leading threads, so
+// waves run full and each pass gets the register budget of its own branch
+// (the surface branch alone fits three waves per SIMD).  Every pixel is
+// written by exactly one thread of one pass.
+__global__ void __launch_bounds__(256) k_composite_sky(Consts k, CompositeArgs a) {
+    __shared__ int list[256];
+    __shared__ int wave_counts[4];
+    const long npix = (long)a.width * a.height;
+    const long groups = (npix + 255) / 256;
+    for (long g = blockIdx.x; g < groups; g += gridDim.x) {
+        const long base = g * 256;
+        const long i = base + threadIdx.x;
+        const bool sky = i < npix && a.sid[i] < 0;
+        const int count = compact_block(sky, list, wave_counts);
+        if ((int)threadIdx.x < count) {
+            const long j = base + list[threadIdx.x];
+            f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height,
+                                a.right, a.up, a.forward, a.thf);
+            f3 c = composite_sky(k, a.tlut, a.svlut, ray, ld3(a.scene, j), ld3(a.space, j),
+                                 a.cam_alt, a.sky_alt, a.sun, a.disk, a.ar, a.thf, a.height, a.c0,
+                                 a.c1, a.bottom);
+            st3(a.hdr, j, maxs(c, 0.0f));
+        }
+        __syncthreads();  // `list` is rewritten by the next group
+    }
+}
+
+// Three waves per SIMD (at most 168 VGPRs, no spills) measured 15 % faster
+// than the compiler's unconstrained two-wave allocation at 1080p.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
+k_composite_surface(Consts k, CompositeArgs a) {
+    __shared__ int list[256];
+    __shared__ int wave_counts[4];
+    const long npix = (long)a.width * a.height;
+    const long groups = (npix + 255) / 256;
+    for (long g = blockIdx.x; g < groups; g += gridDim.x) {
+        const long base = g * 256;
+        const long i = base + threadIdx.x;
+        const bool surface = i < npix && a.sid[i] >= 0;
+        const int count = compact_block(surface, list, wave_counts);
+        if ((int)threadIdx.x < count) {
+            const long j = base + list[threadIdx.x];
+            f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height,
+                                a.right, a.up, a.forward, a.thf);
+            float sd = norm(ld3(a.pos, j));
+            f3 scat = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun,
+                                        a.irradiance, a.bottom, a.ar);
+            f3 trans = surface_transmittance(k, a.tlut, a.cam_alt, ray, sd, a.bottom);
+            st3(a.hdr, j, maxs(scat + ld3(a.scene, j) * trans, 0.0f));
+        }
+        __syncthreads();
+    }
+}
+
+// Surface pixels listed in list[1 .. list[0]] (pixel indices): the plain
+// surface branch of the COMPOSITE view.  Used for the surrogate composite's
+// out-of-domain pixels.
+__global__ void __launch_bounds__(256) k_composite_list(Consts k, CompositeArgs a,
+                                                        const int* __restrict__ list) {
+    const int count = list[0];
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < count; e += gridDim.x * blockDim.x) {
+        const long j = list[1 + e];
+        f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height, a.right,
+                            a.up, a.forward, a.thf);
+        float sd = norm(ld3(a.pos, j));
+        f3 scat = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun, a.irradiance,
+                                    a.bottom, a.ar);
+        f3 trans = surface_transmittance(k, a.tlut, a.cam_alt, ray, sd, a.bottom);
+        st3(a.hdr, j, maxs(scat + ld3(a.scene, j) * trans, 0.0f));
+    }
+}
+
+// Diagnostic views 1..10 of _composite (renderer.py:1941-2055).
+__global__ void __launch_bounds__(256) k_composite_diagnostic(Consts k, CompositeArgs a) {
+    const long npix = (long)a.width * a.height;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    const int px = (int)(i / a.height), py = (int)(i % a.height);
+    float su, sv;
+    f3 ray = pixel_ray(a, px, py, su, sv);
+    const bool surface = a.sid[i] >= 0;
+    f3 color = mk(0.0f, 0.0f, 0.0f);
+    switch (a.view) {
+        case ATMO_VIEW_SKY_VIEW:
+            color = sample_sky_view(k, a.svlut, ray, a.sun, a.sky_alt, a.bottom);
+            break;
+        case ATMO_VIEW_CAMERA_TRANSMITTANCE:
+            color = camera_transmittance(k, a.tlut, a.cam_alt, ray, a.bottom);
+            break;
+        case ATMO_VIEW_TRANSMITTANCE_LUT: {
+            int lx = min((int)(su * (float)k.tw), k.tw - 1);
+            int ly = min((int)(sv * (float)k.th), k.th - 1);
+            color = ld3(a.tlut, (long)lx * k.th + ly);
+            break;
+        }
+        case ATMO_VIEW_MULTI_SCATTERING_LUT: {
+            int lx = min((int)(su * (float)k.mw), k.mw - 1);
+            int ly = min((int)(sv * (float)k.mh), k.mh - 1);
+            color = ld3(a.mslut, (long)lx * k.mh + ly);
+            break;
+        }
+        case ATMO_VIEW_AERIAL_SCATTERING:
+        case ATMO_VIEW_AERIAL_DIRECT_ONLY:
+            if (surface)
+                color = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, norm(ld3(a.pos, i)),
+                                          a.sun, a.irradiance, a.bottom, a.ar);
+            break;
+        case ATMO_VIEW_AERIAL_FROXEL_ONLY:
+            if (surface)
+                color = sample_aerial(k, a.aplut, su, sv, norm(ld3(a.pos, i)), a.cam_alt, ray,
+                                      a.bottom);
+            break;
+        case ATMO_VIEW_AERIAL_BLEND_WEIGHT: {
+            float w = surface ? 1.0f : 0.0f;
+            color = mk(w, w, w);
+            break;
+        }
+        case ATMO_VIEW_AERIAL_TRANSMITTANCE:
+            if (surface)
+                color = surface_transmittance(k, a.tlut, a.cam_alt, ray, norm(ld3(a.pos, i)),
+                                              a.bottom);
+            break;
+        default: {
+            float m = surface ? 1.0f : 0.0f;
+            color = mk(m, m, m);
+            break;
+        }
+    }
+    st3(a.hdr, i, maxs(color, 0.0f));
+}
+
+
+// ------------------------------------------------------------ launchers
+
+constexpr int kGroupT = 8;   // lanes per transmittance texel
+constexpr int kGroupSV = 8;  // lanes per sky-view texel
+
+inline unsigned grid_for(long threads) { return (unsigned)((threads + 255) / 256); }
+
+// Persistent grid of the composite passes: enough 256-thread blocks for
+// several per CU on 256 CUs, never more than there are 256-pixel groups.
+inline unsigned composite_grid(long npix) {
+    const long groups = (npix + 255) / 256;
+    return (unsigned)(groups < 2048 ? groups : 2048);
+}
+
+void launch_transmittance(const Consts& k, float bottom, float* tlut, hipStream_t s) {
+    hipLaunchKernelGGL(k_transmittance<kGroupT>, dim3(grid_for((long)k.tw * k.th * kGroupT)),
+                       dim3(256), 0, s, k, bottom, tlut);
+}
+
+void launch_multi_scattering(const Consts& k, float bottom, f3 irradiance, float ar,
+                             const float* tlut, float* mslut, hipStream_t s) {
+    hipLaunchKernelGGL(k_multi_scattering, dim3(grid_for((long)k.mw * k.mh * 64)), dim3(256), 0, s,
+                       k, bottom, irradiance, ar, tlut, mslut);
+}
+
+void launch_sky_view(const Consts& k, float bottom, float cam_alt, float sun_zc, f3 irradiance,
+                     float ar, const float* tlut, const float* mslut, float* svlut,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_sky_view<kGroupSV>, dim3(grid_for((long)k.sw * k.sh * kGroupSV)),
+                       dim3(256), 0, s, k, bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut,
+                       svlut);
+}
+
+void launch_aerial(const Consts& k, int width, int height, float bottom, float cam_alt, f3 sun,
+                   f3 irradiance, float ar, f3 right, f3 up, f3 forward, float thf,
+                   const float* tlut, const float* mslut, float* ap, hipStream_t s) {
+    hipLaunchKernelGGL(k_aerial, dim3(grid_for((long)k.aw * k.ah * 64)), dim3(256), 0, s, k, width,
+                       height, bottom, cam_alt, sun, irradiance, ar, right, up, forward, thf, tlut,
+                       mslut, ap);
+}
+
+void launch_prepare(const Consts& k, long npix, const float* position_view,
+                    const int32_t* surface_id, float bottom, float cam_alt, f3 sun, float ar,
+                    f3 right, f3 up, f3 forward, const float* tlut, const float* mslut,
+                    float* sun_t, float* sky_l, hipStream_t s) {
+    hipLaunchKernelGGL(k_prepare, dim3(grid_for(npix)), dim3(256), 0, s, k, npix, position_view,
+                       surface_id, bottom, cam_alt, sun, ar, right, up, forward, tlut, mslut, sun_t,
+                       sky_l);
+}
+
+void launch_composite_sky(const Consts& k, const CompositeArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_composite_sky, dim3(composite_grid((long)a.width * a.height)), dim3(256), 0,
+                       s, k, a);
+}
+
+void launch_composite(const Consts& k, const CompositeArgs& a, hipStream_t s) {
+    const long npix = (long)a.width * a.height;
+    if (a.view == ATMO_VIEW_COMPOSITE) {
+        hipLaunchKernelGGL(k_composite_sky, dim3(composite_grid(npix)), dim3(256), 0, s, k, a);
+        hipLaunchKernelGGL(k_composite_surface, dim3(composite_grid(npix)), dim3(256), 0, s, k, a);
+    } else {
+        hipLaunchKernelGGL(k_composite_diagnostic, dim3(grid_for(npix)), dim3(256), 0, s, k, a);
+    }
+}
+
+void launch_composite_list(const Consts& k, const CompositeArgs& a, const int* list,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_composite_list, dim3(256), dim3(256), 0, s, k, a, list);
+}
+
+#if !ATMO_ROBUST
+void launch_batch(const Consts& k, const BatchState* states, unsigned count, int width, int height,
+                  float* const luts[4], const long strides[4], hipStream_t s) {
+    hipLaunchKernelGGL(k_transmittance_batched<kGroupT>,
+                       dim3(grid_for((long)k.tw * k.th * kGroupT), count), dim3(256), 0, s, states,
+                       luts[0], strides[0]);
+    hipLaunchKernelGGL(k_multi_scattering_batched, dim3(grid_for((long)k.mw * k.mh * 64), count),
+                       dim3(256), 0, s, states, (const float*)luts[0], strides[0], luts[1],
+                       strides[1]);
+    hipLaunchKernelGGL(k_sky_view_batched<kGroupSV>,
+                       dim3(grid_for((long)k.sw * k.sh * kGroupSV), count), dim3(256), 0, s, states,
+                       (const float*)luts[0], strides[0], (const float*)luts[1], strides[1],
+                       luts[2], strides[2]);
+    hipLaunchKernelGGL(k_aerial_batched, dim3(grid_for((long)k.aw * k.ah * 64), count), dim3(256), 0,
+                       s, states, width, height, (const float*)luts[0], strides[0],
+                       (const float*)luts[1], strides[1], luts[3], strides[3]);
+}
+#endif
+
+}  // inline namespace ATMO_PRECISION
+}  // namespace atmo

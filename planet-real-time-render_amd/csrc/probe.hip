@@ -1,8 +1,10 @@
-// probe.hip -- throughput probe for the roofline of the atmosphere kernels
-// (libatmo_probe.so).  Measures the sustained rate of v_exp_f32, the
-// transcendental-class instruction that bounds the LUT builds and the
-// composite, with independent dependency chains per lane so the issue rate
-// rather than the latency is timed.
+// probe.hip -- measurement probes for the roofline of the atmosphere kernels
+// (libatmo_probe.so).  atmo_probe_exp_rate measures the sustained rate of
+// v_exp_f32, the transcendental-class instruction that bounds the LUT
+// builds and the composite, with independent dependency chains per lane so
+// the issue rate rather than the latency is timed.  atmo_probe_copy is a
+// plain device copy of known size: the reference point that scales the HBM
+// traffic counters (FETCH_SIZE, WRITE_SIZE) of the other kernels.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -24,9 +26,28 @@ __global__ void __launch_bounds__(256) k_exp_chains(float* __restrict__ out, int
     out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// Grid-stride copy of n 16-byte elements.
+__global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src,
+                                              float4* __restrict__ dst, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
 
 extern "C" {
+
+// Copies `bytes` (a multiple of 16) from src to dst on `stream` with
+// k_copy: exactly `bytes` read and `bytes` written.  Returns 0 on success,
+// otherwise a hipError_t value (hipErrorInvalidValue for a bad size).
+int32_t atmo_probe_copy(const void* src, void* dst, int64_t bytes, void* stream) {
+    if (!src || !dst || bytes <= 0 || bytes % 16 != 0) return (int32_t)hipErrorInvalidValue;
+    const long n = (long)(bytes / 16);
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, (hipStream_t)stream,
+                       (const float4*)src, (float4*)dst, n);
+    return (int32_t)hipGetLastError();
+}
 
 // Runs `blocks` x 256 threads, each `iters` x 8 exp2 evaluations, on the
 // null stream; writes the achieved exp2 rate (per second) to *rate.

@@ -57,7 +57,8 @@ struct AtmoMlp {
     float* wbuf;     // gathered loss weights [B, 6]
     float* partial;  // weight-gradient partial sums [B / kChunk, 64 * 64 + 64]
     float* loss;     // [B / 256 + 1] block partials, loss at [0] after reduce
-    uint16_t* frag;  // bf16 MFMA weight fragments for inference
+    uint16_t* frag;  // bf16 MFMA weight fragments of layers 2-4 for inference
+    float* frag32;   // fp32 MFMA weight fragments of layer 1
     float* bias;     // fp32 biases in inference order [64 + 64 + 64 + 16]
     AtmoSurrogateTransform transform;
     char error[512];
@@ -87,13 +88,33 @@ __device__ __forceinline__ void sample_geometry(float mu, float mu_s, float phi,
     sun = mk(ss * cr_cos(phi), mu_s, ss * cr_sin(phi));
 }
 
+// Cosine of the local horizon seen from altitude h above a planet of
+// radius R: -sqrt(h (2R + h)) / (R + h).
+__device__ __forceinline__ float horizon_cosine(float R, float h) {
+    h = fmaxf(h, 0.0f);
+    return -sqrtf(h * (2.0f * R + h)) / (R + h);
+}
+
+// Input features in [-1, 1]: log-stretched altitude, view and sun zenith
+// cosines (with cosine_stretch s > 0 measured from the local horizon
+// cosine c_h and stretched as asinh((c - c_h) / s) / asinh(2 / s), so the
+// horizon kink sits at the same feature value at every altitude), relative
+// azimuth, log segment length.
 __device__ __forceinline__ void encode(const AtmoSurrogateTransform& t, float h, float mu,
                                        float mu_s, float phi, float d, float x[5]) {
     x[0] = log1pf(fmaxf(h, 0.0f) / t.altitude_scale_m) *
                (2.0f / log1pf(t.altitude_max_m / t.altitude_scale_m)) -
            1.0f;
-    x[1] = mu;
-    x[2] = mu_s;
+    if (t.cosine_stretch > 0.0f) {
+        const float ch = horizon_cosine(t.planet_radius_m, h);
+        const float inv = 1.0f / t.cosine_stretch;
+        const float norm = 1.0f / asinhf(2.0f * inv);
+        x[1] = asinhf((mu - ch) * inv) * norm;
+        x[2] = asinhf((mu_s - ch) * inv) * norm;
+    } else {
+        x[1] = mu;
+        x[2] = mu_s;
+    }
     x[3] = phi * (2.0f / 3.14159265358979f) - 1.0f;
     float ld0 = logf(t.distance_min_m), ld1 = logf(t.distance_max_m);
     x[4] = (logf(d) - ld0) * (2.0f / (ld1 - ld0)) - 1.0f;
@@ -302,19 +323,21 @@ __device__ __forceinline__ bf16x4 pack_bf16(floatx4 v) {
 }
 
 __host__ __device__ constexpr int frag_index(int layer, int mt, int kt) {
-    // layer 0: 4 x 1, layers 1-2: 4 x 4, layer 3: 1 x 4 fragments
-    return layer == 0 ? mt : layer == 1 ? 4 + mt * 4 + kt : layer == 2 ? 20 + mt * 4 + kt : 36 + kt;
+    // layers 1-2: 4 x 4, layer 3: 1 x 4 bf16 fragments (layer 0 runs in fp32)
+    return layer == 1 ? mt * 4 + kt : layer == 2 ? 16 + mt * 4 + kt : 32 + kt;
 }
-constexpr int kFragments = 40;
+constexpr int kFragments = 36;
+constexpr int kFragments32 = 8;  // layer 0: 4 row tiles x 2 K-steps of 4
 
-// Packs fp32 parameters into per-lane bf16 A-operand fragments of
+// Packs fp32 parameters into per-lane A-operand fragments.  Layers 1-3,
 // v_mfma_f32_16x16x16bf16_1k: for tile (mt, kt) lane l holds
-// W[16 mt + l % 16][16 kt + 4 (l / 16) + i], i = 0..3 (zero outside W).
-// Also gathers the biases in output-row order.
+// W[16 mt + l % 16][16 kt + 4 (l / 16) + i], i = 0..3, in bf16 (zero outside
+// W).  Layer 0, v_mfma_f32_16x16x4f32: lane l holds W[16 mt + l % 16][4 kt +
+// l / 16] in fp32.  Also gathers the biases in output-row order.
 __global__ void k_pack_fragments(const float* __restrict__ params, uint16_t* __restrict__ frag,
-                                 float* __restrict__ bias) {
+                                 float* __restrict__ frag32, float* __restrict__ bias) {
     const int lane = threadIdx.x;  // 64 threads
-    for (int layer = 0; layer < 4; ++layer) {
+    for (int layer = 1; layer < 4; ++layer) {
         const Layer L = kLayers[layer];
         const int mts = (L.out + 15) / 16, kts = (L.in + 15) / 16;
         for (int mt = 0; mt < mts; ++mt)
@@ -328,11 +351,16 @@ __global__ void k_pack_fragments(const float* __restrict__ params, uint16_t* __r
                 }
             }
     }
-    for (int e = lane; e < 64 * 3 + 16; e += 64) {
-        const int layer = e / 64 < 3 ? e / 64 : 3;
-        const int row = e - 64 * layer;
+    const Layer L0 = kLayers[0];
+    for (int mt = 0; mt < 4; ++mt)
+        for (int kt = 0; kt < 2; ++kt) {
+            const int row = 16 * mt + lane % 16, col = 4 * kt + lane / 16;
+            frag32[(mt * 2 + kt) * 64 + lane] = col < L0.in ? params[L0.w + row * L0.in + col] : 0.0f;
+        }
+    for (int layer = 0; layer < 4; ++layer) {
         const Layer L = kLayers[layer];
-        bias[e] = row < L.out ? params[L.b + row] : 0.0f;
+        for (int r = lane; r < (layer < 3 ? 64 : 16); r += 64)
+            bias[64 * layer + r] = r < L.out ? params[L.b + r] : 0.0f;
     }
 }
 
@@ -353,36 +381,37 @@ __device__ __forceinline__ bf16x4 silu_bf16(floatx4 c) {
 }
 
 // Runs the MLP for the 64 pixels of a wave.  `x` holds the lane's own
-// pixel features; returns the 6 raw outputs of the lane's pixel.  Layer 1
-// (5 -> 64) runs in FP32 on the VALU: bf16 inputs would quantise log(d) by
-// ~0.4 %, i.e. the segment length by ~3 %.  Layers 2-4 run on MFMA with the
-// weights as the A operand and activations as the B operand, so the output
-// tile of one layer (lane l: neurons 4 (l / 16) + i of pixel l % 16) is
-// exactly the B fragment of the next layer's K-step: no data movement.
-// w1 holds W1 row-major (64 x 5), bias the fp32 biases of all layers.
+// pixel features; returns the 6 raw outputs of the lane's pixel.  The wave
+// walks its pixels 16 at a time.  Layer 1 (5 -> 64) runs on
+// v_mfma_f32_16x16x4f32 in full fp32 (bf16 inputs would quantise log(d) by
+// ~0.4 %, i.e. the segment length by ~3 %): lane l feeds feature
+// 4 kt + l / 16 of pixel l % 16 as the B operand.  Layers 2-4 run on
+// v_mfma_f32_16x16x16bf16_1k with the weights as the A operand and the
+// activations as the B operand, so the output tile of one layer (lane l:
+// neurons 4 (l / 16) + i of pixel l % 16) is exactly the B fragment of the
+// next layer's K-step: no data movement between layers.  `a32` holds the
+// lane's 8 fp32 layer-1 fragments, `bias` the fp32 biases of all layers.
 __device__ __forceinline__ void mlp_wave(const float x[5], const bf16x4* __restrict__ frag,
-                                         const float* __restrict__ w1,
+                                         const float a32[kFragments32],
                                          const float* __restrict__ bias, float out[6]) {
     const int lane = threadIdx.x % 64;
     const int group = lane / 16, col = lane % 16;
 #pragma unroll 1
     for (int j = 0; j < 4; ++j) {
         const int src = 16 * j + col;
-        float f[5];
-#pragma unroll
-        for (int q = 0; q < 5; ++q) f[q] = __shfl(x[q], src, 64);
+        const float f0 = __shfl(x[0], src, 64), f1 = __shfl(x[1], src, 64);
+        const float f2 = __shfl(x[2], src, 64), f3 = __shfl(x[3], src, 64);
+        const float f4 = __shfl(x[4], src, 64);
+        const float b0 = group == 0 ? f0 : group == 1 ? f1 : group == 2 ? f2 : f3;
+        const float b1 = group == 0 ? f4 : 0.0f;
         bf16x4 act[4];
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
             floatx4 c;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int n = 16 * mt + 4 * group + i;
-                float acc = bias[n];
-#pragma unroll
-                for (int q = 0; q < 5; ++q) acc += f[q] * w1[n * 5 + q];
-                c[i] = acc;
-            }
+            for (int i = 0; i < 4; ++i) c[i] = bias[16 * mt + 4 * group + i];
+            c = __builtin_amdgcn_mfma_f32_16x16x4f32(a32[2 * mt], b0, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x4f32(a32[2 * mt + 1], b1, c, 0, 0, 0);
             act[mt] = silu_bf16(c);
         }
 #pragma unroll
@@ -420,6 +449,11 @@ __device__ __forceinline__ void mlp_wave(const float x[5], const bf16x4* __restr
     }
 }
 
+// Network outputs -> (in-scatter per unit irradiance L, transmittance T).
+// v = y * out_scale + out_mean.  Mode 1 (linear): L = max(v, 0),
+// T = clamp(v, 0, 1).  Mode 0 (log): L = exp(v) - eps_inscatter,
+// T = exp(v) - eps_transmittance.  Mode 2 (floored log): L = exp(v), zero
+// below 1.5 eps_inscatter (the training floor), T as in mode 0.
 __device__ __forceinline__ void decode(const AtmoSurrogateTransform& t, const float raw[6], f3& L,
                                        f3& T) {
     float y[6];
@@ -430,8 +464,14 @@ __device__ __forceinline__ void decode(const AtmoSurrogateTransform& t, const fl
         T = clamp3(mk(y[3], y[4], y[5]), 0.0f, 1.0f);
         return;
     }
-    L = mk(fmaxf(expf(y[0]) - t.eps_inscatter, 0.0f), fmaxf(expf(y[1]) - t.eps_inscatter, 0.0f),
-           fmaxf(expf(y[2]) - t.eps_inscatter, 0.0f));
+    float l[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const float e = expf(y[q]);
+        l[q] = t.output_mode == 2 ? (e < 1.5f * t.eps_inscatter ? 0.0f : e)
+                                  : fmaxf(e - t.eps_inscatter, 0.0f);
+    }
+    L = mk(l[0], l[1], l[2]);
     T = clamp3(mk(expf(y[3]) - t.eps_transmittance, expf(y[4]) - t.eps_transmittance,
                   expf(y[5]) - t.eps_transmittance),
                0.0f, 1.0f);
@@ -453,50 +493,42 @@ struct SurrogateArgs {
     f3 right, up, forward;
     float thf, c0, c1;
     const uint16_t* frag;
-    const float* w1;
+    const float* frag32;
     const float* bias;
     AtmoSurrogateTransform t;
 };
 
-// The COMPOSITE view with the surrogate in place of the surface branch's
-// in-scatter and transmittance.  Persistent blocks: the 20 KiB of weight
-// fragments are staged in LDS once per block, which then walks 256-pixel
-// groups.  In a group, sky pixels are finished by their own thread (the
-// plain sky branch); surface pixels are compacted onto the leading threads,
-// so the network runs in full waves over surface pixels only (the MFMA chain
-// needs the whole wave; a wave runs it when any lane holds an in-domain
-// pixel).  Out-of-domain surface pixels are queued for
-// k_composite_fallback, which keeps the plain integrator's registers out of
-// this kernel.
+// The surface pixels of the COMPOSITE view with the surrogate in place of
+// the surface branch's in-scatter (and, unless exact_transmittance is set,
+// transmittance); sky pixels are left to the plain sky pass.  Persistent
+// blocks: the 18 KiB of bf16 weight fragments are staged in LDS once per
+// block, which then walks 256-pixel groups.  In a group, surface pixels are
+// compacted onto the leading threads, so the network runs in full waves
+// over surface pixels only (the MFMA chain needs the whole wave; a wave runs
+// it when any lane holds an in-domain pixel).  Out-of-domain surface pixels
+// are queued for k_composite_fallback, which keeps the plain integrator's
+// registers out of this kernel.
 __global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, SurrogateArgs a,
                                                              int* __restrict__ queue) {
     __shared__ bf16x4 frag_s[kFragments * 64];
     __shared__ float bias_s[64 * 3 + 16];
-    __shared__ float w1_s[64 * 5];
     __shared__ int list[256];
     __shared__ int wave_counts[4];
     const bf16x4* fg = reinterpret_cast<const bf16x4*>(a.frag);
     for (int e = threadIdx.x; e < kFragments * 64; e += blockDim.x) frag_s[e] = fg[e];
     for (int e = threadIdx.x; e < 64 * 3 + 16; e += blockDim.x) bias_s[e] = a.bias[e];
-    for (int e = threadIdx.x; e < 64 * 5; e += blockDim.x) w1_s[e] = a.w1[e];
     // (the first compaction's barriers order this staging before any read)
+    const int lane = threadIdx.x % 64;
+    float a32[kFragments32];
+#pragma unroll
+    for (int f = 0; f < kFragments32; ++f) a32[f] = a.frag32[f * 64 + lane];
 
     const long npix = (long)a.width * a.height;
     const long groups = (npix + 255) / 256;
-    const int lane = threadIdx.x % 64;
     for (long g = blockIdx.x; g < groups; g += gridDim.x) {
         const long base = g * 256;
         const long i = base + threadIdx.x;
-        const bool live = i < npix;
-        const bool surface = live && a.sid[i] >= 0;
-        if (live && !surface) {
-            f3 ray = camera_ray((int)(i / a.height), (int)(i % a.height), a.width, a.height,
-                                a.right, a.up, a.forward, a.thf);
-            f3 c = composite_sky(k, a.tlut, a.svlut, ray, ld3(a.scene, i), ld3(a.space, i),
-                                 a.cam_alt, a.sky_alt, a.sun, a.disk, a.ar, a.thf, a.height, a.c0,
-                                 a.c1, a.bottom);
-            st3(a.hdr, i, maxs(c, 0.0f));
-        }
+        const bool surface = i < npix && a.sid[i] >= 0;
         const int count = compact_block(surface, list, wave_counts);
         const bool has = (int)threadIdx.x < count;
         const long j = base + (has ? list[threadIdx.x] : 0);
@@ -523,10 +555,12 @@ __global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, Surrogate
             encode(a.t, a.cam_alt, clampf(ray.y, -1.0f, 1.0f), clampf(a.sun.y, -1.0f, 1.0f), phi,
                    fmaxf(sd, a.t.distance_min_m), x);
             float raw[6];
-            mlp_wave(x, frag_s, w1_s, bias_s, raw);
+            mlp_wave(x, frag_s, a32, bias_s, raw);
             if (learned) {
                 f3 L, T;
                 decode(a.t, raw, L, T);
+                if (a.t.exact_transmittance)
+                    T = surface_transmittance(k, a.tlut, a.cam_alt, ray, sd, a.bottom);
                 st3(a.hdr, j, maxs(L * a.irradiance + ld3(a.scene, j) * T, 0.0f));
             }
         }
@@ -536,44 +570,28 @@ __global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, Surrogate
 
 __global__ void k_zero_queue(int* __restrict__ queue) { queue[0] = 0; }
 
-// Out-of-domain surface pixels of the surrogate composite: the plain
-// surface branch (renderer.py:1863-1881).
-__global__ void __launch_bounds__(256) k_composite_fallback(Consts k, SurrogateArgs a,
-                                                            const int* __restrict__ queue) {
-    const int count = queue[0];
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < count; e += gridDim.x * blockDim.x) {
-        const long j = queue[1 + e];
-        const f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height,
-                                  a.right, a.up, a.forward, a.thf);
-        const float sd = norm(ld3(a.pos, j));
-        f3 L = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun, a.irradiance,
-                                 a.bottom, a.ar);
-        f3 T = surface_transmittance(k, a.tlut, a.cam_alt, ray, sd, a.bottom);
-        st3(a.hdr, j, maxs(L + ld3(a.scene, j) * T, 0.0f));
-    }
-}
-
 // Batch inference on encoded features (testing and evaluation): x [n, 5]
 // -> y [n, 6] raw network outputs, through the same MFMA path.
 __global__ void __launch_bounds__(256) k_mlp_infer(long n, const float* __restrict__ x,
                                                    const uint16_t* __restrict__ frag,
-                                                   const float* __restrict__ w1,
+                                                   const float* __restrict__ frag32,
                                                    const float* __restrict__ bias,
                                                    float* __restrict__ y) {
     __shared__ bf16x4 frag_s[kFragments * 64];
     __shared__ float bias_s[64 * 3 + 16];
-    __shared__ float w1_s[64 * 5];
     const bf16x4* fg = reinterpret_cast<const bf16x4*>(frag);
     for (int e = threadIdx.x; e < kFragments * 64; e += blockDim.x) frag_s[e] = fg[e];
     for (int e = threadIdx.x; e < 64 * 3 + 16; e += blockDim.x) bias_s[e] = bias[e];
-    for (int e = threadIdx.x; e < 64 * 5; e += blockDim.x) w1_s[e] = w1[e];
     __syncthreads();
+    const int lane = threadIdx.x % 64;
+    float a32[kFragments32];
+    for (int f = 0; f < kFragments32; ++f) a32[f] = frag32[f * 64 + lane];
     const long i_raw = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long i = i_raw < n ? i_raw : n - 1;
     float f[5];
     for (int q = 0; q < 5; ++q) f[q] = x[i * 5 + q];
     float out[6];
-    mlp_wave(f, frag_s, w1_s, bias_s, out);
+    mlp_wave(f, frag_s, a32, bias_s, out);
     if (i_raw < n)
         for (int q = 0; q < 6; ++q) y[i * 6 + q] = out[q];
 }
@@ -589,7 +607,7 @@ hipError_t alloc(T** p, size_t count) {
 void free_mlp(AtmoMlp* m) {
     void* bufs[] = {m->params, m->grads, m->adam_m, m->adam_v, m->z[0], m->z[1], m->z[2], m->z[3],
                     m->a[0], m->a[1], m->a[2], m->dz[0], m->dz[1], m->dz[2], m->dz[3], m->xbuf,
-                    m->wbuf, m->partial, m->loss, m->frag, m->bias};
+                    m->wbuf, m->partial, m->loss, m->frag, m->frag32, m->bias};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
 }
@@ -655,6 +673,7 @@ int32_t atmo_mlp_create(int32_t max_batch, AtmoMlp** out) {
               alloc(&m->partial, chunks * (64 * 64 + 64)) == hipSuccess &&
               alloc(&m->loss, B * 6 / 256 + 2) == hipSuccess &&
               alloc(&m->frag, (size_t)kFragments * 64 * 4) == hipSuccess &&
+              alloc(&m->frag32, (size_t)kFragments32 * 64) == hipSuccess &&
               alloc(&m->bias, (size_t)64 * 3 + 16) == hipSuccess;
     for (int l = 0; l < 4 && ok; ++l) {
         ok = alloc(&m->z[l], B * kLayers[l].out) == hipSuccess &&
@@ -672,7 +691,10 @@ int32_t atmo_mlp_create(int32_t max_batch, AtmoMlp** out) {
     }
     m->transform.altitude_max_m = 100000.0f;
     m->transform.altitude_scale_m = 100.0f;
-    m->transform.output_mode = 1;
+    m->transform.output_mode = 2;
+    m->transform.eps_inscatter = 1.0e-7f;
+    m->transform.eps_transmittance = 1.0e-6f;
+    m->transform.planet_radius_m = 6360000.0f;
     m->transform.distance_min_m = 1.0f;
     m->transform.distance_max_m = 1.0e6f;
     for (int q = 0; q < 6; ++q) m->transform.out_scale[q] = 1.0f;
@@ -700,7 +722,8 @@ int32_t atmo_mlp_set_params(AtmoMlp* m, const float* host_params, int64_t count)
         hipMemset(m->adam_m, 0, count * 4) != hipSuccess ||
         hipMemset(m->adam_v, 0, count * 4) != hipSuccess)
         return ATMO_EHIP;
-    hipLaunchKernelGGL(k_pack_fragments, dim3(1), dim3(64), 0, 0, m->params, m->frag, m->bias);
+    hipLaunchKernelGGL(k_pack_fragments, dim3(1), dim3(64), 0, 0, m->params, m->frag, m->frag32,
+                       m->bias);
     if (status(m, "atmo_mlp_set_params") != ATMO_OK) return ATMO_EHIP;
     return hipDeviceSynchronize() == hipSuccess ? ATMO_OK : ATMO_EHIP;
 }
@@ -717,7 +740,9 @@ int32_t atmo_mlp_set_transform(AtmoMlp* m, const AtmoSurrogateTransform* t) {
     if (!mlp_ok(m)) return ATMO_EHANDLE;
     if (!t || !(t->altitude_max_m > 0.0f) || !(t->altitude_scale_m > 0.0f) ||
         !(t->distance_min_m > 0.0f) ||
-        !(t->distance_max_m > t->distance_min_m) || t->output_mode < 0 || t->output_mode > 1)
+        !(t->distance_max_m > t->distance_min_m) || t->output_mode < 0 || t->output_mode > 2 ||
+        !(t->planet_radius_m > 0.0f) || !(t->cosine_stretch >= 0.0f) ||
+        (t->exact_transmittance != 0 && t->exact_transmittance != 1))
         return ATMO_EINVAL;
     m->transform = *t;
     return ATMO_OK;
@@ -781,7 +806,7 @@ int32_t atmo_mlp_train_step(AtmoMlp* m, const float* x, const float* target, con
 int32_t atmo_mlp_pack(AtmoMlp* m, void* stream) {
     if (!mlp_ok(m)) return ATMO_EHANDLE;
     hipLaunchKernelGGL(k_pack_fragments, dim3(1), dim3(64), 0, (hipStream_t)stream, m->params, m->frag,
-                       m->bias);
+                       m->frag32, m->bias);
     return status(m, "atmo_mlp_pack");
 }
 
@@ -789,7 +814,7 @@ int32_t atmo_mlp_infer(AtmoMlp* m, const float* x, int64_t n, float* y, void* st
     if (!mlp_ok(m)) return ATMO_EHANDLE;
     if (!x || !y || n <= 0) return ATMO_ESHAPE;
     hipLaunchKernelGGL(k_mlp_infer, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, n, x,
-                       m->frag, m->params + kLayers[0].w, m->bias, y);
+                       m->frag, m->frag32, m->bias, y);
     return status(m, "atmo_mlp_infer");
 }
 
@@ -854,18 +879,44 @@ int32_t atmo_composite_surrogate(AtmoHandle* h, AtmoMlp* m, const float* scene_h
     a.c0 = c0;
     a.c1 = c1;
     a.frag = m->frag;
-    a.w1 = m->params + kLayers[0].w;
+    a.frag32 = m->frag32;
     a.bias = m->bias;
     a.t = m->transform;
     const long npix = (long)h->width * h->height;
     const long groups = (npix + 255) / 256;
     hipStream_t s = (hipStream_t)stream;
+    CompositeArgs ca;
+    ca.scene = scene_hdr;
+    ca.space = space_hdr;
+    ca.sid = surface_id;
+    ca.pos = position_view;
+    ca.hdr = hdr;
+    ca.tlut = h->tlut;
+    ca.mslut = h->mslut;
+    ca.svlut = h->svlut;
+    ca.aplut = h->aplut;
+    ca.width = h->width;
+    ca.height = h->height;
+    ca.bottom = bottom;
+    ca.cam_alt = cam_alt;
+    ca.sky_alt = sky_alt;
+    ca.sun = a.sun;
+    ca.irradiance = a.irradiance;
+    ca.disk = a.disk;
+    ca.ar = ar;
+    ca.right = a.right;
+    ca.up = a.up;
+    ca.forward = a.forward;
+    ca.thf = thf;
+    ca.c0 = c0;
+    ca.c1 = c1;
+    ca.view = ATMO_VIEW_COMPOSITE;
+    if (atmo_launch_composite_sky(h, ca, s) != ATMO_OK) return ATMO_EHIP;
     hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, s, h->queue);
     hipLaunchKernelGGL(k_composite_surrogate, dim3((unsigned)(groups < 1024 ? groups : 1024)),
                        dim3(256), 0, s, h->k, a, h->queue);
-    hipLaunchKernelGGL(k_composite_fallback, dim3(256), dim3(256), 0, s, h->k, a,
-                       (const int*)h->queue);
-    return atmo_launch_status(h, "atmo_composite_surrogate");
+    if (atmo_launch_status(h, "atmo_composite_surrogate") != ATMO_OK) return ATMO_EHIP;
+    return atmo_launch_composite_list(h, ca, h->queue, s);
 }
 
 }  // extern "C"

@@ -67,12 +67,22 @@ def _device_tensor(value, name: str, shape: tuple[int, ...], dtype, device):
     return tensor
 
 
+PRECISIONS = {"reference": _capi.PRECISION_REFERENCE, "robust": _capi.PRECISION_ROBUST}
+
+
 class AtmosphereRenderer:
     """Spherical Rayleigh/Mie atmosphere rendered through compact LUTs
     (renderer.py:42-116).  The camera sits at the floating origin; the planet
-    centre is (0, -camera_radius, 0) in the camera's East-Up-North frame."""
+    centre is (0, -camera_radius, 0) in the camera's East-Up-North frame.
 
-    def __init__(self, width: int, height: int, config: AtmosphereConfig, device=None):
+    `precision` "reference" (default) reproduces the reference's float32
+    kernels; "robust" evaluates the planet-scale cancellations (near-tangent
+    ray-sphere solves, the sky-view horizon split, endpoints above 1000 km)
+    in float64 and compensates the optical-depth sums -- an extension with
+    no counterpart in the reference."""
+
+    def __init__(self, width: int, height: int, config: AtmosphereConfig, device=None,
+                 precision: str = "reference"):
         torch = _torch()
         if not torch.cuda.is_available():
             raise RuntimeError("AtmosphereRenderer needs a HIP device (libatmo.so has no CPU path)")
@@ -82,6 +92,9 @@ class AtmosphereRenderer:
         self.device = torch.device(device if device is not None else "cuda", )
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}")
+        self.precision = precision
         lib = library()
         self._lib = lib
         self._params = config.to_params()
@@ -90,6 +103,7 @@ class AtmosphereRenderer:
             check(lib.atmo_create(ctypes.byref(self._params), self.width, self.height,
                                   ctypes.byref(handle)))
             self._handle = handle
+            check(lib.atmo_set_precision(handle, PRECISIONS[precision]), handle)
             c = config
             shapes = {
                 _capi.BUF_TRANSMITTANCE: (c.transmittance_lut_width, c.transmittance_lut_height, 3),
@@ -331,4 +345,4 @@ class AtmosphereRenderer:
                         float(star_contrast_end), int(diagnostic_view))
 
 
-__all__ = ["AtmosphereRenderer", "DeviceField"]
+__all__ = ["AtmosphereRenderer", "DeviceField", "PRECISIONS"]

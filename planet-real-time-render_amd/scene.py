@@ -213,6 +213,51 @@ def make_frame(width: int = 1920, height: int = 1080, config_index: int = 1,
     )
 
 
+STRESS_ALTITUDES_M = (0.5, 2_000.0, 100_000.0, 400_000.0, 2_000_000.0)
+TANGENT_BAND_DEG = 0.01
+
+
+def limb_depression_deg(planet_radius_m: float, altitude_m: float) -> float:
+    """Angle of the planet's limb below the local horizontal seen from
+    altitude h: acos(R / (R + h))."""
+    return math.degrees(math.acos(planet_radius_m / (planet_radius_m + altitude_m)))
+
+
+def make_stress_frame(altitude_m: float, width: int = 3840, height: int = 2160,
+                      band_fraction: float = 0.20, max_distance_m: float = 3.0e6,
+                      seed_offset: int = 0) -> FrameInputs:
+    """Planet-scale precision stress frame: the view is centred on the
+    planet's limb with a vertical FOV of 2 x 0.01 deg / band_fraction, so
+    `band_fraction` of the pixel rows look within 0.01 deg of tangency;
+    surface ranges logU[10 m, max_distance_m]; sun zenith U[0, 100] deg,
+    yaw U[0, 360) deg; 30 % sky pixels."""
+    g = np.random.Generator(np.random.PCG64(SEED_BASE + 4 + 1000 * seed_offset))
+    config = full_frame_config()
+    R = EARTH_RADIUS_M
+    sun = sun_from_angles(g.uniform(0.0, 100.0), g.uniform(0.0, 2.0 * math.pi))
+    yaw = g.uniform(0.0, 360.0)
+    pitch = -limb_depression_deg(R, altitude_m)
+    basis = view_basis_local(yaw, pitch)
+    vfov_deg = 2.0 * TANGENT_BAND_DEG / band_fraction
+    thf = math.tan(math.radians(vfov_deg) / 2.0)
+    sky = g.random((width, height)) < 0.30
+    distance = np.exp(g.uniform(math.log(10.0), math.log(max_distance_m), size=(width, height)))
+    rays = pixel_rays_view(width, height, thf)
+    position_view = (rays * distance[:, :, None]).astype(np.float32)
+    surface_id = np.where(sky, -1, 0).astype(np.int32)
+    scene = g.random((width, height, 3)).astype(np.float32)
+    scene[sky] = np.asarray(SKY_SCENE_HDR, dtype=np.float32)
+    space = np.zeros((width, height, 3), dtype=np.float32)
+    stars = sky & (g.random((width, height)) < 0.001)
+    space[stars] = g.uniform(0.0, 10.0, size=(int(stars.sum()), 3)).astype(np.float32)
+    return FrameInputs(
+        config=config, width=width, height=height, planet_radius_m=R,
+        camera_radius_m=R + altitude_m, sun_local=sun,
+        lighting=LightingState(sun, SUN_ANGULAR_RADIUS_DEG, SOLAR_IRRADIANCE),
+        view_basis=basis, tangent_half_fov=thf, position_view=position_view,
+        surface_id=surface_id, scene_hdr=scene, space_hdr=space)
+
+
 def lut_chain_state(seed_offset: int = 0):
     """LUT-chain state: camera at 2 km, sun zenith U[0, 100] deg."""
     g = np.random.Generator(np.random.PCG64(SEED_BASE + 0 + 1000 * seed_offset))
@@ -220,8 +265,59 @@ def lut_chain_state(seed_offset: int = 0):
     return EARTH_RADIUS_M, EARTH_RADIUS_M + 2_000.0, sun
 
 
+SMALL_PLANET_RADIUS_M = 50_000.0
+
+
+def batch_states(count: int = 256, seed_offset: int = 0, config: AtmosphereConfig | None = None):
+    """`count` independent states for the batched LUT build.  Per state: sun
+    zenith U[0, 100] deg, sun azimuth U[0, 2pi), camera altitude
+    logU[1 m, 2000 km], yaw U[0, 360) deg, pitch U[-20, 20] deg, and the
+    Rayleigh and Mie coefficients each scaled by their own U[0.7, 1.3]
+    factor.  The second half of the states sits on a 50 km planet with a
+    10 km shell, scale heights and the ozone tent divided by 10."""
+    from .batch import AtmosphereState
+
+    g = np.random.Generator(np.random.PCG64(SEED_BASE + 2 + 1000 * seed_offset))
+    base = config if config is not None else full_frame_config()
+    thf = math.tan(math.radians(VERTICAL_FOV_DEG) / 2.0)
+    states = []
+    for i in range(count):
+        small = i >= count // 2
+        sun = sun_from_angles(g.uniform(0.0, 100.0), g.uniform(0.0, 2.0 * math.pi))
+        altitude = float(math.exp(g.uniform(math.log(1.0), math.log(2.0e6))))
+        yaw, pitch = g.uniform(0.0, 360.0), g.uniform(-20.0, 20.0)
+        kr, km = g.uniform(0.7, 1.3), g.uniform(0.7, 1.3)
+        cfg = base.replace(
+            rayleigh_scattering_per_m=tuple(kr * v for v in base.rayleigh_scattering_per_m),
+            mie_scattering_per_m=tuple(km * v for v in base.mie_scattering_per_m),
+            mie_extinction_per_m=tuple(km * v for v in base.mie_extinction_per_m),
+        )
+        radius = EARTH_RADIUS_M
+        if small:
+            radius = SMALL_PLANET_RADIUS_M
+            cfg = cfg.replace(
+                top_altitude_m=base.top_altitude_m / 10.0,
+                rayleigh_scale_height_m=base.rayleigh_scale_height_m / 10.0,
+                mie_scale_height_m=base.mie_scale_height_m / 10.0,
+                absorption_peak_altitude_m=base.absorption_peak_altitude_m / 10.0,
+                absorption_half_width_m=base.absorption_half_width_m / 10.0,
+            )
+        states.append(AtmosphereState(
+            config=cfg,
+            planet_radius_m=radius,
+            camera_radius_m=radius + altitude,
+            sun_local=sun,
+            solar_irradiance=SOLAR_IRRADIANCE,
+            sun_angular_radius_degrees=SUN_ANGULAR_RADIUS_DEG,
+            view_basis=view_basis_local(yaw, pitch),
+            tangent_half_fov=thf,
+        ))
+    return states
+
+
 __all__ = [
     "EARTH_RADIUS_M",
+    "batch_states",
     "FrameInputs",
     "SOLAR_IRRADIANCE",
     "SUN_ANGULAR_RADIUS_DEG",
@@ -229,6 +325,7 @@ __all__ = [
     "lut_chain_config",
     "lut_chain_state",
     "make_frame",
+    "make_stress_frame",
     "pixel_rays_view",
     "small_planet_config",
     "sun_from_angles",

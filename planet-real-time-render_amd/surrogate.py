@@ -11,9 +11,16 @@ linear in the solar irradiance (the multi-scattering LUT is too), so a
 Everything numeric runs in libatmo.so: labels from the GPU integrator
 (atmo_surrogate_labels), FP32 forward / backward / Adam training kernels,
 BF16 MFMA inference.  This module drives them and handles the target
-transform (in-scatter and transmittance standardised per output, linear by
-default, logarithmic as an option) and the weight file (raw float32
-parameters + JSON metadata).
+transform and the weight file (raw float32 parameters + JSON metadata).
+
+Default transform: the view and sun cosines are encoded relative to the
+local horizon and asinh-stretched, the in-scatter is learned as
+log(max(L, 1e-7)) (relative accuracy wherever L can show in a pixel; the
+night side collapses onto the floor), the transmittance as log(T + 1e-6),
+and the surrogate composite evaluates the transmittance exactly with the
+plain two-sample formula (about 4 % of the plain surface cost) because the
+reference's reversed-ray transmittance has kinks a 64-wide network does not
+fit to 1 %.
 """
 
 from __future__ import annotations
@@ -46,9 +53,22 @@ WEIGHT_PIXEL_FLOOR = 5.0e-3
 WEIGHT_CLIP = 1.0e3
 EPS_INSCATTER = 1.0e-8
 EPS_TRANSMITTANCE = 1.0e-6
+# floor of the floored-log in-scatter target: 1e-7 per unit irradiance is
+# below 1e-6 of a daylight pixel
+INSCATTER_FLOOR = 1.0e-7
+COSINE_STRETCH = 0.02
 OUTPUT_LOG = 0
 OUTPUT_LINEAR = 1
+OUTPUT_LOG_FLOOR = 2
 SEED_BASE = 20261203
+MODE_NAMES = {OUTPUT_LOG: "log", OUTPUT_LINEAR: "linear", OUTPUT_LOG_FLOOR: "log_floor"}
+WEIGHTS_DIR = Path(__file__).resolve().parent / "weights"
+
+
+def default_weights_path() -> Path:
+    """The shipped network for the full-frame atmosphere (Earth radius
+    6360 km, 100 km shell with ozone, 0.266 deg sun): <path>.bin + .json."""
+    return WEIGHTS_DIR / "earth_aerial_surrogate"
 
 
 def _torch():
@@ -75,30 +95,38 @@ def sample_inputs(n: int, seed: int) -> np.ndarray:
     return raw[g.permutation(n)].astype(np.float32)
 
 
-def targets_from_labels(labels: np.ndarray, mode: int = OUTPUT_LINEAR) -> np.ndarray:
-    """Training targets before standardisation: the labels themselves
-    (in-scatter >= 0, transmittance clamped to [0, 1]) in linear mode,
-    log(L + eps) and log(T + eps) in log mode."""
+def targets_from_labels(labels: np.ndarray, mode: int = OUTPUT_LOG_FLOOR) -> np.ndarray:
+    """Training targets before standardisation.  Linear: the labels
+    themselves (in-scatter >= 0, transmittance clamped to [0, 1]).  Log:
+    log(L + eps) and log(T + eps).  Floored log: log(max(L, floor)) and
+    log(T + eps)."""
     labels = np.asarray(labels, dtype=np.float64)
     out = np.empty_like(labels)
     if mode == OUTPUT_LINEAR:
         out[:, :3] = np.maximum(labels[:, :3], 0.0)
         out[:, 3:] = np.clip(labels[:, 3:], 0.0, 1.0)
         return out
-    out[:, :3] = np.log(np.maximum(labels[:, :3], 0.0) + EPS_INSCATTER)
+    if mode == OUTPUT_LOG_FLOOR:
+        out[:, :3] = np.log(np.maximum(labels[:, :3], INSCATTER_FLOOR))
+    else:
+        out[:, :3] = np.log(np.maximum(labels[:, :3], 0.0) + EPS_INSCATTER)
     out[:, 3:] = np.log(np.clip(labels[:, 3:], 0.0, 1.0) + EPS_TRANSMITTANCE)
     return out
 
 
 def decode_outputs(raw: np.ndarray, mean: np.ndarray, scale: np.ndarray,
-                   mode: int = OUTPUT_LINEAR):
+                   mode: int = OUTPUT_LOG_FLOOR):
     """Network outputs -> (L per unit irradiance, T), the inverse of
     targets_from_labels after standardisation (host mirror of the kernel's
     decode)."""
     y = np.asarray(raw, dtype=np.float64) * scale + mean
     if mode == OUTPUT_LINEAR:
         return np.maximum(y[:, :3], 0.0), np.clip(y[:, 3:], 0.0, 1.0)
-    L = np.maximum(np.exp(y[:, :3]) - EPS_INSCATTER, 0.0)
+    e = np.exp(y[:, :3])
+    if mode == OUTPUT_LOG_FLOOR:
+        L = np.where(e < 1.5 * INSCATTER_FLOOR, 0.0, e)
+    else:
+        L = np.maximum(e - EPS_INSCATTER, 0.0)
     T = np.clip(np.exp(y[:, 3:]) - EPS_TRANSMITTANCE, 0.0, 1.0)
     return L, T
 
@@ -130,15 +158,29 @@ def loss_weights(labels: np.ndarray, scale: np.ndarray, mode: int = OUTPUT_LINEA
     return (w / w.mean()).astype(np.float32)
 
 
-def encode_host(raw: np.ndarray) -> np.ndarray:
+def horizon_cosine(planet_radius_m: float, altitude_m):
+    """Cosine of the local horizon's zenith angle at altitude h:
+    -sqrt(h (2R + h)) / (R + h)."""
+    h = np.maximum(np.asarray(altitude_m, dtype=np.float64), 0.0)
+    return -np.sqrt(h * (2.0 * planet_radius_m + h)) / (planet_radius_m + h)
+
+
+def encode_host(raw: np.ndarray, planet_radius_m: float = 6_360_000.0,
+                cosine_stretch: float = COSINE_STRETCH) -> np.ndarray:
     """Host mirror of the kernel's input normalisation (float64)."""
     raw = np.asarray(raw, dtype=np.float64)
     lo, hi = math.log(DISTANCE_RANGE_M[0]), math.log(DISTANCE_RANGE_M[1])
     x = np.empty_like(raw)
     x[:, 0] = np.log1p(np.maximum(raw[:, 0], 0.0) / ALTITUDE_SCALE_M) * (
         2.0 / math.log1p(ALTITUDE_MAX_M / ALTITUDE_SCALE_M)) - 1.0
-    x[:, 1] = raw[:, 1]
-    x[:, 2] = raw[:, 2]
+    if cosine_stretch > 0.0:
+        ch = horizon_cosine(planet_radius_m, raw[:, 0])
+        norm = 1.0 / math.asinh(2.0 / cosine_stretch)
+        x[:, 1] = np.arcsinh((raw[:, 1] - ch) / cosine_stretch) * norm
+        x[:, 2] = np.arcsinh((raw[:, 2] - ch) / cosine_stretch) * norm
+    else:
+        x[:, 1] = raw[:, 1]
+        x[:, 2] = raw[:, 2]
     x[:, 3] = raw[:, 3] * (2.0 / math.pi) - 1.0
     x[:, 4] = (np.log(raw[:, 4]) - lo) * (2.0 / (hi - lo)) - 1.0
     return x
@@ -183,7 +225,10 @@ class SurrogateMLP:
         self.max_batch = int(max_batch)
         self.out_mean = np.zeros(6)
         self.out_scale = np.ones(6)
-        self.output_mode = OUTPUT_LINEAR
+        self.output_mode = OUTPUT_LOG_FLOOR
+        self.planet_radius_m = 6_360_000.0
+        self.cosine_stretch = COSINE_STRETCH
+        self.exact_transmittance = True
         self.metadata: dict = {}
         self.set_params(init_params(0))
         self._push_transform()
@@ -221,9 +266,12 @@ class SurrogateMLP:
         t.distance_min_m, t.distance_max_m = DISTANCE_RANGE_M
         t.out_mean[:] = [float(v) for v in self.out_mean]
         t.out_scale[:] = [float(v) for v in self.out_scale]
-        t.eps_inscatter = EPS_INSCATTER
+        t.eps_inscatter = INSCATTER_FLOOR if self.output_mode == OUTPUT_LOG_FLOOR else EPS_INSCATTER
         t.eps_transmittance = EPS_TRANSMITTANCE
         t.output_mode = int(self.output_mode)
+        t.planet_radius_m = float(self.planet_radius_m)
+        t.cosine_stretch = float(self.cosine_stretch)
+        t.exact_transmittance = 1 if self.exact_transmittance else 0
         return t
 
     def _push_transform(self) -> None:
@@ -248,9 +296,22 @@ class SurrogateMLP:
         self.out_mean = np.asarray(mean, dtype=np.float64)
         self.out_scale = np.asarray(scale, dtype=np.float64)
         if mode is not None:
-            if mode not in (OUTPUT_LOG, OUTPUT_LINEAR):
-                raise ValueError(f"output mode must be {OUTPUT_LOG} or {OUTPUT_LINEAR}, got {mode}")
+            if mode not in (OUTPUT_LOG, OUTPUT_LINEAR, OUTPUT_LOG_FLOOR):
+                raise ValueError(f"unknown output mode {mode}")
             self.output_mode = int(mode)
+        self._push_transform()
+
+    def set_encoding(self, planet_radius_m: float, cosine_stretch: float) -> None:
+        """Planet radius of the horizon-relative cosine encoding and its
+        stretch (0 = raw cosines)."""
+        self.planet_radius_m = float(planet_radius_m)
+        self.cosine_stretch = float(cosine_stretch)
+        self._push_transform()
+
+    def set_exact_transmittance(self, exact: bool) -> None:
+        """True: the surrogate composite evaluates the transmittance with
+        the plain two-sample formula; False: it uses the network's."""
+        self.exact_transmittance = bool(exact)
         self._push_transform()
 
     # ------------------------------------------------------------- compute
@@ -313,7 +374,11 @@ class SurrogateMLP:
             "layers": [[5, 64], [64, 64], [64, 64], [64, 6]],
             "activation": "silu",
             "params": _capi.MLP_PARAMS,
-            "output_mode": "linear" if self.output_mode == OUTPUT_LINEAR else "log",
+            "output_mode": MODE_NAMES[self.output_mode],
+            "planet_radius_m": self.planet_radius_m,
+            "cosine_stretch": self.cosine_stretch,
+            "exact_transmittance": self.exact_transmittance,
+            "inscatter_floor": INSCATTER_FLOOR,
             "out_mean": [float(v) for v in self.out_mean],
             "out_scale": [float(v) for v in self.out_scale],
             "altitude_max_m": ALTITUDE_MAX_M,
@@ -333,7 +398,10 @@ class SurrogateMLP:
         params = np.fromfile(path.with_suffix(".bin"), dtype="<f4")
         net = cls(max_batch=max_batch, device=device)
         net.set_params(params)
-        mode = OUTPUT_LINEAR if meta.get("output_mode", "log") == "linear" else OUTPUT_LOG
+        mode = {v: k for k, v in MODE_NAMES.items()}[meta.get("output_mode", "log")]
+        net.planet_radius_m = float(meta.get("planet_radius_m", 6_360_000.0))
+        net.cosine_stretch = float(meta.get("cosine_stretch", 0.0))
+        net.exact_transmittance = bool(meta.get("exact_transmittance", False))
         net.set_output_transform(meta["out_mean"], meta["out_scale"], mode)
         net.metadata = meta
         return net
@@ -361,13 +429,16 @@ def generate_labels(renderer, raw_host: np.ndarray, planet_radius_m: float,
 def train(renderer, planet_radius_m: float, sun_angular_radius_deg: float,
           samples: int = 1 << 22, batch: int = 65536, epochs: int = 100, lr: float = 1.0e-3,
           seed: int = SEED_BASE + 3, schedule: str = "constant", lr_final: float = 1.0e-5,
-          weighted: bool = True, mode: int = OUTPUT_LINEAR,
+          weighted: bool = False, mode: int = OUTPUT_LOG_FLOOR,
+          cosine_stretch: float = COSINE_STRETCH, exact_transmittance: bool = True,
           log=None) -> tuple[SurrogateMLP, TrainingReport]:
     """Label `samples` seeded inputs with the GPU integrator and train the
     MLP with Adam for `epochs` passes in shuffled batches.  `schedule`
     "constant" keeps lr; "cosine" anneals it from lr to lr_final.  With
-    `weighted`, each output's squared error is weighted as in loss_weights;
-    `mode` picks the linear or log output transform."""
+    `weighted`, each output's squared error is weighted as in loss_weights
+    (linear and log modes); `mode` picks the output transform,
+    `cosine_stretch` the input encoding and `exact_transmittance` what the
+    surrogate composite uses for T."""
     torch = _torch()
     device = renderer.device
     t0 = time.perf_counter()
@@ -381,8 +452,10 @@ def train(renderer, planet_radius_m: float, sun_angular_radius_deg: float,
     scale = np.where(scale > 1e-6, scale, 1.0)
     t_std = torch.from_numpy(((targets - mean) / scale).astype(np.float32)).to(device)
     weight = (torch.from_numpy(loss_weights(labels_host, scale, mode)).to(device)
-              if weighted else None)
+              if weighted and mode != OUTPUT_LOG_FLOOR else None)
     net = SurrogateMLP(max_batch=batch, device=device)
+    net.set_encoding(planet_radius_m, cosine_stretch)
+    net.set_exact_transmittance(exact_transmittance)
     net.set_output_transform(mean, scale, mode)
     net.set_params(init_params(seed))
     x = net.encode(raw)
@@ -414,7 +487,7 @@ def train(renderer, planet_radius_m: float, sun_angular_radius_deg: float,
     net.metadata.update({"samples": samples, "batch": batch, "epochs": epochs, "lr": lr,
                          "schedule": schedule, "lr_final": lr_final if schedule == "cosine" else lr,
                          "weighted_loss": weighted, "seed": seed,
-                         "output_mode": "linear" if mode == OUTPUT_LINEAR else "log",
+                         "output_mode": MODE_NAMES[mode],
                          "planet_radius_m": planet_radius_m,
                          "sun_angular_radius_deg": sun_angular_radius_deg,
                          "final_loss": report.losses[-1][1] if report.losses else None})
@@ -425,7 +498,9 @@ __all__ = [
     "SurrogateMLP",
     "TrainingReport",
     "decode_outputs",
+    "default_weights_path",
     "encode_host",
+    "horizon_cosine",
     "generate_labels",
     "init_params",
     "loss_weights",

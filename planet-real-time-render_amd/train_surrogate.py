@@ -18,7 +18,6 @@ import time
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
-DEFAULT_OUT = HERE / "weights" / "earth_aerial_surrogate"
 
 
 def main(argv=None) -> int:
@@ -30,11 +29,16 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--schedule", choices=("constant", "cosine"), default="cosine")
     ap.add_argument("--lr-final", type=float, default=1.0e-5)
-    ap.add_argument("--unweighted", action="store_true",
-                    help="plain MSE instead of the pixel-share weighted loss")
-    ap.add_argument("--log-targets", action="store_true",
-                    help="log-space outputs instead of linear standardised ones")
-    ap.add_argument("--out", type=Path, default=DEFAULT_OUT)
+    ap.add_argument("--mode", choices=("log_floor", "log", "linear"), default="log_floor",
+                    help="output transform (default: floored log)")
+    ap.add_argument("--weighted", action="store_true",
+                    help="pixel-share weighted loss (linear and log modes)")
+    ap.add_argument("--cosine-stretch", type=float, default=None,
+                    help="horizon-relative asinh stretch of the cosines (0: raw cosines)")
+    ap.add_argument("--network-transmittance", action="store_true",
+                    help="composite with the network's transmittance instead of the exact one")
+    ap.add_argument("--out", type=Path, default=None,
+                    help="output path without suffix (default: the shipped weights)")
     args = ap.parse_args(argv)
 
     sys.path.insert(0, str(HERE.parent))
@@ -53,9 +57,14 @@ def main(argv=None) -> int:
     net, report = sur.train(renderer, scene.EARTH_RADIUS_M, scene.SUN_ANGULAR_RADIUS_DEG,
                             samples=args.samples, batch=args.batch, epochs=args.epochs,
                             lr=args.lr, schedule=args.schedule, lr_final=args.lr_final,
-                            weighted=not args.unweighted,
-                            mode=sur.OUTPUT_LOG if args.log_targets else sur.OUTPUT_LINEAR,
+                            weighted=args.weighted,
+                            mode={v: k for k, v in sur.MODE_NAMES.items()}[args.mode],
+                            cosine_stretch=(sur.COSINE_STRETCH if args.cosine_stretch is None
+                                            else args.cosine_stretch),
+                            exact_transmittance=not args.network_transmittance,
                             log=log, **kwargs)
+    if args.out is None:
+        args.out = sur.default_weights_path()
     args.out.parent.mkdir(parents=True, exist_ok=True)
     net.metadata["training_seconds"] = report.seconds
     net.metadata["label_seconds"] = report.label_seconds

@@ -29,12 +29,13 @@ def _dev(a, device):
     return torch.from_numpy(np.ascontiguousarray(a)).to(device)
 
 
-def gpu_frame(pkg, inputs, device, view=0):
+def gpu_frame(pkg, inputs, device, view=0, precision="reference"):
     """The reference host sequence update -> prepare_frame -> composite on
     libatmo.so; returns every LUT and output as NumPy."""
     import torch
 
-    r = pkg.AtmosphereRenderer(inputs.width, inputs.height, inputs.config, device)
+    r = pkg.AtmosphereRenderer(inputs.width, inputs.height, inputs.config, device,
+                               precision=precision)
     pos, sid = _dev(inputs.position_view, device), _dev(inputs.surface_id, device)
     scene_hdr, space_hdr = _dev(inputs.scene_hdr, device), _dev(inputs.space_hdr, device)
     hdr = torch.full_like(scene_hdr, float("nan"))

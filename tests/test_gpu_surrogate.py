@@ -78,6 +78,19 @@ def test_mfma_inference_matches_quantised_fp64(sur, device):
     net.close()
 
 
+def test_device_encoding_matches_host(sur, device):
+    """atmo_mlp_encode (the composite's encoding) against the float64 host
+    mirror, horizon-relative cosines included."""
+    net = sur.SurrogateMLP(max_batch=4096, device=device)
+    raw = sur.sample_inputs(4096, 11)
+    x = net.encode(_dev(raw, device)).cpu().numpy()
+    want = sur.encode_host(raw, net.planet_radius_m, net.cosine_stretch)
+    err = np.abs(x - want).max()
+    print(f"encoding: max abs err {err:.2e}", flush=True)
+    assert err < 2e-5
+    net.close()
+
+
 def test_fp32_forward_matches_fp64(sur, device):
     net = sur.SurrogateMLP(max_batch=4096, device=device)
     params = _random_params(3)
@@ -178,10 +191,11 @@ def test_labels_match_cpu_integrator(pkg, scene, sur, ref32, device):
 def test_training_reduces_loss_and_weights_round_trip(pkg, scene, sur, device, tmp_path):
     r = pkg.AtmosphereRenderer(16, 16, scene.full_frame_config(), device)
     net, report = sur.train(r, scene.EARTH_RADIUS_M, scene.SUN_ANGULAR_RADIUS_DEG,
-                            samples=1 << 16, batch=4096, epochs=20)
+                            samples=1 << 16, batch=4096, epochs=60)
     first, last = report.losses[0][1], report.losses[-1][1]
-    print(f"training: loss {first:.3e} -> {last:.3e} in {report.seconds:.2f} s", flush=True)
-    assert last < 0.7 * first
+    print(f"training: loss {first:.3e} (epoch 10) -> {last:.3e} (epoch 60) in "
+          f"{report.seconds:.2f} s", flush=True)
+    assert last < 0.6 * first
     path = tmp_path / "weights"
     net.save(path)
     again = sur.SurrogateMLP.load(path, max_batch=4096, device=device)

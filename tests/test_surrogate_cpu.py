@@ -57,12 +57,36 @@ def test_input_normalisation_and_target_transform(pkg):
     assert np.all(np.abs(x) <= 1.0 + 1e-6)
     assert x[:, 4].min() < -0.99 and x[:, 4].max() > 0.99  # log-uniform distance spans the range
     labels = np.column_stack([np.geomspace(1e-7, 0.5, 50)] * 3 + [np.linspace(0.0, 1.0, 50)] * 3)
-    for mode in (s.OUTPUT_LINEAR, s.OUTPUT_LOG):
+    for mode in (s.OUTPUT_LINEAR, s.OUTPUT_LOG, s.OUTPUT_LOG_FLOOR):
         t = s.targets_from_labels(labels, mode)
         mean, scale = t.mean(axis=0), t.std(axis=0)
         L, T = s.decode_outputs((t - mean) / scale, mean, scale, mode)
-        np.testing.assert_allclose(L, labels[:, :3], rtol=1e-9, atol=1e-15)
+        want = labels[:, :3]
+        if mode == s.OUTPUT_LOG_FLOOR:  # values below 1.5 x the floor decode to 0
+            want = np.where(want < 1.5 * s.INSCATTER_FLOOR, 0.0, want)
+        np.testing.assert_allclose(L, want, rtol=1e-9, atol=1e-15)
         np.testing.assert_allclose(T, labels[:, 3:], rtol=1e-9, atol=1e-12)
+
+
+def test_horizon_relative_cosine_encoding(pkg):
+    """The local horizon cosine equals -sqrt(1 - (R / (R + h))^2) (tangent
+    from a sphere), and a cosine on the horizon encodes to 0 at every
+    altitude; encoding is monotonic in the cosine and stays in [-1, 1]."""
+    import importlib
+
+    s = importlib.import_module(pkg.__name__ + ".surrogate")
+    R = 6_360_000.0
+    h = np.array([0.0, 0.5, 2_000.0, 100_000.0, 2.0e6])
+    np.testing.assert_allclose(s.horizon_cosine(R, h), -np.sqrt(1.0 - (R / (R + h)) ** 2),
+                               rtol=1e-9, atol=1e-15)
+    raw = np.column_stack([h, s.horizon_cosine(R, h), s.horizon_cosine(R, h),
+                           np.full(5, 1.0), np.full(5, 100.0)])
+    x = s.encode_host(raw, R)
+    np.testing.assert_allclose(x[:, 1:3], 0.0, atol=1e-12)
+    mu = np.linspace(-1.0, 1.0, 2001)
+    raw = np.column_stack([np.full_like(mu, 300.0), mu, mu, np.zeros_like(mu), np.ones_like(mu)])
+    x = s.encode_host(raw, R)
+    assert np.all(np.diff(x[:, 1]) > 0.0) and np.all(np.abs(x[:, 1:3]) <= 1.0)
 
 
 def test_linear_loss_weights_measure_relative_pixel_error(pkg):
