@@ -150,6 +150,21 @@ enum {
 };
 int32_t atmo_set_precision(AtmoHandle* handle, int32_t precision);
 
+/* Launch-shape autotuner (no counterpart in the reference).  The
+ * transmittance and sky-view builds march each texel with a group of G
+ * lanes (G = 4, 8 or 16; default 8) that combine their step chunks;
+ * atmo_autotune times every G of both builds on `stream` with this
+ * handle's LUT sizes and the given arguments (as the handle's next update
+ * would pass them), keeps the fastest, and leaves T, MS and SV built with
+ * the chosen shapes.  It synchronises the stream.  chosen (may be NULL)
+ * receives {G_T, G_SV}.  Different G reassociate the optical-depth sum and
+ * the sky-view scan: results move by float32 rounding only. */
+int32_t atmo_autotune(AtmoHandle* handle, float bottom_radius, const float irradiance[3],
+                      float angular_radius, float camera_altitude, float sun_zenith_cosine,
+                      void* stream, int32_t chosen[2]);
+int32_t atmo_get_tuning(AtmoHandle* handle, int32_t tuning[2]);
+int32_t atmo_set_tuning(AtmoHandle* handle, const int32_t tuning[2]);
+
 /* Replaces `_build_transmittance(bottom_radius)` (renderer.py:1034-1063). */
 int32_t atmo_build_transmittance(AtmoHandle* handle, float bottom_radius,
                                  void* stream);
@@ -246,6 +261,11 @@ const char* atmo_batch_last_error(const AtmoBatch* batch);
 /* Builds all four LUTs of every state: one graph launch on `stream` (the
  * graph is captured on the first call). */
 int32_t atmo_batch_run(AtmoBatch* batch, void* stream);
+/* Times the nine (G_T, G_SV) launch shapes of the batch's four stages on
+ * `stream` (synchronising it), keeps the fastest for the captured graph
+ * (capturing again on the next atmo_batch_run if it changed) and writes it
+ * to chosen (may be NULL).  The LUT arrays hold the last trial's results. */
+int32_t atmo_batch_autotune(AtmoBatch* batch, void* stream, int32_t chosen[2]);
 /* Device address and shape of the batch's LUT array `which`
  * (ATMO_BUF_TRANSMITTANCE .. ATMO_BUF_AERIAL_SCATTERING): shape[0] = count,
  * then the single-state LUT shape. */

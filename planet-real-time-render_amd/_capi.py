@@ -38,6 +38,9 @@ EXPORTED_SYMBOLS = (
     "atmo_get_buffer",
     "atmo_bind_buffer",
     "atmo_set_precision",
+    "atmo_autotune",
+    "atmo_get_tuning",
+    "atmo_set_tuning",
     "atmo_build_transmittance",
     "atmo_build_multi_scattering",
     "atmo_build_sky_view",
@@ -48,6 +51,7 @@ EXPORTED_SYMBOLS = (
     "atmo_batch_destroy",
     "atmo_batch_last_error",
     "atmo_batch_run",
+    "atmo_batch_autotune",
     "atmo_batch_get_buffer",
     "atmo_mlp_create",
     "atmo_mlp_destroy",
@@ -155,6 +159,10 @@ def _signatures(lib: ctypes.CDLL) -> None:
     lib.atmo_get_buffer.argtypes = [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int64)]
     lib.atmo_bind_buffer.argtypes = [_P, _I, _P, ctypes.c_int64]
     lib.atmo_set_precision.argtypes = [_P, _I]
+    _PI = ctypes.POINTER(ctypes.c_int32)
+    lib.atmo_autotune.argtypes = [_P, _F, _PF3, _F, _F, _F, _P, _PI]
+    lib.atmo_get_tuning.argtypes = [_P, _PI]
+    lib.atmo_set_tuning.argtypes = [_P, _PI]
     lib.atmo_build_transmittance.argtypes = [_P, _F, _P]
     lib.atmo_build_multi_scattering.argtypes = [_P, _F, _PF3, _F, _P]
     lib.atmo_build_sky_view.argtypes = [_P, _F, _F, _F, _PF3, _F, _P]
@@ -175,6 +183,7 @@ def _signatures(lib: ctypes.CDLL) -> None:
     lib.atmo_batch_last_error.argtypes = [_P]
     lib.atmo_batch_last_error.restype = ctypes.c_char_p
     lib.atmo_batch_run.argtypes = [_P, _P]
+    lib.atmo_batch_autotune.argtypes = [_P, _P, ctypes.POINTER(ctypes.c_int32)]
     lib.atmo_batch_get_buffer.argtypes = [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int64)]
     lib.atmo_mlp_create.argtypes = [_I, ctypes.POINTER(_P)]
     lib.atmo_mlp_destroy.argtypes = [_P]

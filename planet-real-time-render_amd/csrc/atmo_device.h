@@ -716,18 +716,27 @@ __device__ __forceinline__ void scattering_segment(const Consts& k, const float*
 // continuous terminator refinement (renderer.py:901-1032).  The refinement
 // weight and the coarse segment of an interval share one midpoint, so the
 // sun geometry at that point is computed once for both.
-__device__ __forceinline__ f3 surface_inscatter(const Consts& k, const float* __restrict__ tlut,
-                                                const float* __restrict__ mslut, float h0, f3 ray,
-                                                float surface_distance, f3 sun, f3 irradiance,
-                                                float bottom, float ar) {
+//
+// kRefine = false is the coarse-only variant: it returns false as soon as an
+// interval's refinement weight is positive (the pixel then needs the full
+// variant), and otherwise produces exactly the full variant's result, since
+// an interval with weight 0 contributes its coarse segment unchanged.  It
+// carries none of the refinement's registers.
+template <bool kRefine>
+__device__ __forceinline__ bool surface_inscatter_t(const Consts& k, const float* __restrict__ tlut,
+                                                    const float* __restrict__ mslut, float h0,
+                                                    f3 ray, float surface_distance, f3 sun,
+                                                    f3 irradiance, float bottom, float ar,
+                                                    f3& out) {
     f3 radiance = mk(0.0f, 0.0f, 0.0f);
     f3 view_t = mk(1.0f, 1.0f, 1.0f);
+    out = radiance;
     float on, of;
     bool oh;
     shell_roots(k, h0, ray.y, k.top, bottom, on, of, oh);
     float start = fmaxf(on, 0.0f);
     float end = fminf(surface_distance, of);
-    if (!(oh && end > start)) return radiance;
+    if (!(oh && end > start)) return true;
 
     const f3 origin = mk(0.0f, bottom + h0, 0.0f);
     float r0, r1;
@@ -756,6 +765,7 @@ __device__ __forceinline__ f3 surface_inscatter(const Consts& k, const float* __
         float sun_cosine = dot(radial, sun);
         SunGeom g = sun_geometry(k, alt, sun_cosine, bottom, ar);
         float weight = fmaxf(prox, 4.0f * g.visibility * (1.0f - g.visibility));
+        if (!kRefine && weight > 0.0f) return false;
         // coarse _scattering_segment at the same midpoint
         f3 cl, ct;
         {
@@ -771,7 +781,7 @@ __device__ __forceinline__ f3 surface_inscatter(const Consts& k, const float* __
             cl = src * integ;
         }
         f3 il = cl, it = ct;
-        if (weight > 0.0f) {
+        if (kRefine && weight > 0.0f) {
             f3 fl = mk(0.0f, 0.0f, 0.0f);
             f3 ft = mk(1.0f, 1.0f, 1.0f);
             const int m = k.term_sub;
@@ -793,7 +803,18 @@ __device__ __forceinline__ f3 surface_inscatter(const Consts& k, const float* __
         view_t = view_t * it;
         d0 = d1;
     }
-    return maxs(radiance, 0.0f);
+    out = maxs(radiance, 0.0f);
+    return true;
+}
+
+__device__ __forceinline__ f3 surface_inscatter(const Consts& k, const float* __restrict__ tlut,
+                                                const float* __restrict__ mslut, float h0, f3 ray,
+                                                float surface_distance, f3 sun, f3 irradiance,
+                                                float bottom, float ar) {
+    f3 out;
+    surface_inscatter_t<true>(k, tlut, mslut, h0, ray, surface_distance, sun, irradiance, bottom,
+                              ar, out);
+    return out;
 }
 
 // Camera-to-surface transmittance from two unoccluded T-LUT samples along
@@ -942,6 +963,21 @@ __device__ __forceinline__ int compact_block(bool flag, int* list, int* wave_cou
     if (flag) list[offset + __popcll(mask & ((1ull << lane) - 1ull))] = threadIdx.x;
     __syncthreads();
     return total;
+}
+
+// Appends the flagged lanes' pixel indices to the deferred-pixel queue (one
+// atomic per wave, by the lowest flagged lane).  Every lane of the wave
+// must call it.
+__device__ __forceinline__ void enqueue_wave(bool flag, long pixel, int* __restrict__ queue,
+                                             int parity) {
+    const unsigned long long mask = __ballot(flag);
+    if (!mask) return;
+    const int lane = threadIdx.x % 64;
+    const int leader = __ffsll((long long)mask) - 1;
+    int slot = 0;
+    if (lane == leader) slot = atomicAdd(queue + parity, __popcll(mask));
+    slot = __shfl(slot, leader, 64);
+    if (flag) queue[2 + slot + __popcll(mask & ((1ull << lane) - 1ull))] = (int)pixel;
 }
 
 }  // inline namespace ATMO_PRECISION

@@ -25,8 +25,14 @@ struct AtmoHandle {
     float* sun_t;
     float* sky_l;
     bool borrowed[6];  // buffer supplied by the caller through atmo_bind_buffer
-    int* queue;        // surrogate composite: [0] = count, then out-of-domain pixel indices
+    // Pixels deferred by the COMPOSITE passes (refinement, surrogate
+    // fallback): [0], [1] = two counters, then pixel indices.  A call
+    // counts in [parity]; its last pass zeroes [1 - parity] for the next.
+    int* queue;
+    int queue_parity;
     int precision;     // ATMO_PRECISION_REFERENCE or ATMO_PRECISION_ROBUST
+    int group_t;       // lanes per transmittance texel (4, 8, 16), autotuned
+    int group_sv;      // lanes per sky-view texel (4, 8, 16), autotuned
     char error[512];
 };
 
@@ -48,6 +54,8 @@ struct CompositeArgs {
     atmo::f3 right, up, forward;
     float thf, c0, c1;
     int view;
+    int* queue;  // COMPOSITE view: deferred pixels, layout of AtmoHandle::queue
+    int parity;  // the counter of this call: queue[parity]
 };
 
 // Float32 kernel arguments of one state's LUT builds, derived on the host
@@ -67,12 +75,11 @@ struct BatchState {
     float thf;
 };
 
-// Launch the sky pass of the COMPOSITE view, and the plain surface branch
-// for the pixels listed in list[1 .. list[0]], in the handle's precision
-// variant (atmo_kernels.hip).
+// Launch the sky pass of the COMPOSITE view, and the full surface branch
+// for the pixels deferred to a.queue, in the handle's precision variant
+// (atmo_kernels.hip).
 int32_t atmo_launch_composite_sky(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream);
-int32_t atmo_launch_composite_list(AtmoHandle* h, const CompositeArgs& a, const int* list,
-                                   hipStream_t stream);
+int32_t atmo_launch_composite_list(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream);
 
 inline atmo::f3 mk3(const float* a) { return atmo::f3{a[0], a[1], a[2]}; }
 

@@ -232,6 +232,8 @@ int32_t atmo_create(const AtmoParams* params, int32_t width, int32_t height, Atm
     h->k = make_consts(*params);
     h->width = width;
     h->height = height;
+    h->group_t = 8;
+    h->group_sv = 8;
     h->error[0] = 0;
     for (int b = 0; b < 6; ++b) {
         *buffer_slot(h, b) = nullptr;
@@ -254,7 +256,8 @@ int32_t atmo_create(const AtmoParams* params, int32_t width, int32_t height, Atm
         }
     }
     h->queue = nullptr;
-    if (hipMalloc((void**)&h->queue, ((size_t)width * (size_t)height + 1) * sizeof(int)) != hipSuccess) {
+    if (hipMalloc((void**)&h->queue, ((size_t)width * (size_t)height + 2) * sizeof(int)) != hipSuccess ||
+        hipMemset(h->queue, 0, 2 * sizeof(int)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
         set_error(nullptr, "atmo_create: device allocation failed");
         free_buffers(h);
         delete h;
@@ -338,7 +341,7 @@ int32_t atmo_set_precision(AtmoHandle* h, int32_t precision) {
 
 int32_t atmo_build_transmittance(AtmoHandle* h, float bottom, void* stream) {
     if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
-    ATMO_STAGE(h, launch_transmittance, h->k, bottom, h->tlut, (hipStream_t)stream);
+    ATMO_STAGE(h, launch_transmittance, h->k, bottom, h->tlut, (hipStream_t)stream, h->group_t);
     return atmo_launch_status(h, "atmo_build_transmittance");
 }
 
@@ -362,7 +365,7 @@ int32_t atmo_build_sky_view(AtmoHandle* h, float bottom, float cam_alt, float su
         return ATMO_EINVAL;
     }
     ATMO_STAGE(h, launch_sky_view, h->k, bottom, cam_alt, sun_zc, host3(irradiance), ar, h->tlut,
-               h->mslut, h->svlut, (hipStream_t)stream);
+               h->mslut, h->svlut, (hipStream_t)stream, h->group_sv);
     return atmo_launch_status(h, "atmo_build_sky_view");
 }
 
@@ -404,9 +407,8 @@ int32_t atmo_launch_composite_sky(AtmoHandle* h, const CompositeArgs& a, hipStre
     return atmo_launch_status(h, "k_composite_sky");
 }
 
-int32_t atmo_launch_composite_list(AtmoHandle* h, const CompositeArgs& a, const int* list,
-                                   hipStream_t stream) {
-    ATMO_STAGE(h, launch_composite_list, h->k, a, list, stream);
+int32_t atmo_launch_composite_list(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream) {
+    ATMO_STAGE(h, launch_composite_list, h->k, a, stream);
     return atmo_launch_status(h, "k_composite_list");
 }
 
@@ -454,8 +456,103 @@ int32_t atmo_composite(AtmoHandle* h, const float* scene_hdr, const float* space
     a.c0 = c0;
     a.c1 = c1;
     a.view = view;
+    a.queue = h->queue;
+    a.parity = h->queue_parity;
+    if (view == ATMO_VIEW_COMPOSITE) h->queue_parity ^= 1;
     ATMO_STAGE(h, launch_composite, h->k, a, (hipStream_t)stream);
     return atmo_launch_status(h, "atmo_composite");
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ autotuner
+
+namespace {
+
+constexpr int kGroups[3] = {4, 8, 16};
+
+// Mean duration in ms of `launch` on `stream` over `reps` runs after one
+// warm-up run; a negative value if a HIP call failed.
+template <typename F>
+float time_launch(hipStream_t stream, int reps, F&& launch) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1.0f;
+    launch();
+    (void)hipEventRecord(e0, stream);
+    for (int r = 0; r < reps; ++r) launch();
+    (void)hipEventRecord(e1, stream);
+    float ms = -1.0f;
+    if (hipEventSynchronize(e1) == hipSuccess && hipGetLastError() == hipSuccess &&
+        hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+        ms /= (float)reps;
+    else
+        ms = -1.0f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t atmo_autotune(AtmoHandle* h, float bottom, const float irradiance[3], float ar,
+                      float cam_alt, float sun_zc, void* stream, int32_t chosen[2]) {
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
+    if (!irradiance) {
+        set_error(h, "atmo_autotune: NULL irradiance");
+        return ATMO_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const f3 e = host3(irradiance);
+    float best = 1.0e30f;
+    int gt = h->group_t;
+    for (int g : kGroups) {
+        float ms = time_launch(s, 5, [&] { ATMO_STAGE(h, launch_transmittance, h->k, bottom, h->tlut, s, g); });
+        if (ms < 0.0f) return atmo_launch_status(h, "atmo_autotune");
+        if (ms < best) best = ms, gt = g;
+    }
+    h->group_t = gt;
+    ATMO_STAGE(h, launch_transmittance, h->k, bottom, h->tlut, s, gt);
+    ATMO_STAGE(h, launch_multi_scattering, h->k, bottom, e, ar, h->tlut, h->mslut, s);
+    best = 1.0e30f;
+    int gs = h->group_sv;
+    for (int g : kGroups) {
+        float ms = time_launch(s, 5, [&] {
+            ATMO_STAGE(h, launch_sky_view, h->k, bottom, cam_alt, sun_zc, e, ar, h->tlut, h->mslut,
+                       h->svlut, s, g);
+        });
+        if (ms < 0.0f) return atmo_launch_status(h, "atmo_autotune");
+        if (ms < best) best = ms, gs = g;
+    }
+    h->group_sv = gs;
+    // leave the sky-view LUT as the chosen variant builds it
+    ATMO_STAGE(h, launch_sky_view, h->k, bottom, cam_alt, sun_zc, e, ar, h->tlut, h->mslut,
+               h->svlut, s, gs);
+    if (chosen) {
+        chosen[0] = gt;
+        chosen[1] = gs;
+    }
+    return atmo_launch_status(h, "atmo_autotune");
+}
+
+int32_t atmo_get_tuning(AtmoHandle* h, int32_t tuning[2]) {
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
+    if (!tuning) return ATMO_EINVAL;
+    tuning[0] = h->group_t;
+    tuning[1] = h->group_sv;
+    return ATMO_OK;
+}
+
+int32_t atmo_set_tuning(AtmoHandle* h, const int32_t tuning[2]) {
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
+    if (!tuning || !atmo::ref::valid_group(tuning[0]) || !atmo::ref::valid_group(tuning[1])) {
+        set_error(h, "atmo_set_tuning: lanes per texel must be 4, 8 or 16");
+        return ATMO_EINVAL;
+    }
+    h->group_t = tuning[0];
+    h->group_sv = tuning[1];
+    return ATMO_OK;
 }
 
 }  // extern "C"
@@ -468,6 +565,8 @@ struct AtmoBatch {
     int width;
     int height;
     Consts k0;           // LUT dimensions shared by every state
+    int group_t;         // lanes per transmittance texel (autotuned)
+    int group_sv;        // lanes per sky-view texel (autotuned)
     BatchState* states;  // device state table [count]
     float* luts[4];      // T, MS, SV, AP arrays [count * per-state floats]
     bool borrowed[4];    // array supplied by the caller
@@ -575,6 +674,8 @@ int32_t atmo_batch_create(const AtmoBatchState* states, int32_t count, int32_t w
     }
     memset(b, 0, sizeof *b);
     b->magic = kBatchMagic;
+    b->group_t = 8;
+    b->group_sv = 8;
     b->count = count;
     b->width = width;
     b->height = height;
@@ -623,13 +724,14 @@ int32_t atmo_batch_run(AtmoBatch* b, void* stream) {
     if (!b->captured) {
         const Consts& k = b->k0;
         const unsigned n = (unsigned)b->count;
-        if (hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking) != hipSuccess ||
+        if ((!b->capture && hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking) != hipSuccess) ||
             hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             batch_error(b, "atmo_batch_run: cannot start graph capture");
             return ATMO_EHIP;
         }
         hipStream_t cs = b->capture;
-        atmo::ref::launch_batch(k, b->states, n, b->width, b->height, b->luts, b->strides, cs);
+        atmo::ref::launch_batch(k, b->states, n, b->width, b->height, b->luts, b->strides, cs,
+                                b->group_t, b->group_sv);
         hipError_t launch = hipGetLastError();
         hipError_t end = hipStreamEndCapture(cs, &b->graph);
         if (launch != hipSuccess || end != hipSuccess ||
@@ -642,6 +744,40 @@ int32_t atmo_batch_run(AtmoBatch* b, void* stream) {
     if (hipGraphLaunch(b->exec, (hipStream_t)stream) != hipSuccess) {
         batch_error(b, "atmo_batch_run: graph launch failed");
         return ATMO_EHIP;
+    }
+    return ATMO_OK;
+}
+
+int32_t atmo_batch_autotune(AtmoBatch* b, void* stream, int32_t chosen[2]) {
+    if (!batch_ok(b)) return ATMO_EHANDLE;
+    hipStream_t s = (hipStream_t)stream;
+    float best = 1.0e30f;
+    int bt = b->group_t, bs = b->group_sv;
+    for (int gt : kGroups)
+        for (int gs : kGroups) {
+            float ms = time_launch(s, 2, [&] {
+                atmo::ref::launch_batch(b->k0, b->states, (unsigned)b->count, b->width, b->height,
+                                        b->luts, b->strides, s, gt, gs);
+            });
+            if (ms < 0.0f) {
+                batch_error(b, "atmo_batch_autotune: launch failed");
+                return ATMO_EHIP;
+            }
+            if (ms < best) best = ms, bt = gt, bs = gs;
+        }
+    if (bt != b->group_t || bs != b->group_sv) {
+        // the captured graph holds the old shapes: capture again on the next run
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+        b->captured = false;
+        b->group_t = bt;
+        b->group_sv = bs;
+    }
+    if (chosen) {
+        chosen[0] = bt;
+        chosen[1] = bs;
     }
     return ATMO_OK;
 }

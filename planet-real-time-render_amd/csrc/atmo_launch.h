@@ -10,12 +10,12 @@
 #include "atmo_handle.h"
 
 #define ATMO_DECLARE_LAUNCHERS                                                                   \
-    void launch_transmittance(const Consts& k, float bottom, float* tlut, hipStream_t s);        \
+    void launch_transmittance(const Consts& k, float bottom, float* tlut, hipStream_t s, int g);  \
     void launch_multi_scattering(const Consts& k, float bottom, f3 irradiance, float ar,         \
                                  const float* tlut, float* mslut, hipStream_t s);                \
     void launch_sky_view(const Consts& k, float bottom, float cam_alt, float sun_zc,             \
                          f3 irradiance, float ar, const float* tlut, const float* mslut,         \
-                         float* svlut, hipStream_t s);                                           \
+                         float* svlut, hipStream_t s, int g);                                    \
     void launch_aerial(const Consts& k, int width, int height, float bottom, float cam_alt,      \
                        f3 sun, f3 irradiance, float ar, f3 right, f3 up, f3 forward, float thf,  \
                        const float* tlut, const float* mslut, float* ap, hipStream_t s);         \
@@ -25,8 +25,7 @@
                         const float* mslut, float* sun_t, float* sky_l, hipStream_t s);          \
     void launch_composite_sky(const Consts& k, const CompositeArgs& a, hipStream_t s);          \
     void launch_composite(const Consts& k, const CompositeArgs& a, hipStream_t s);              \
-    void launch_composite_list(const Consts& k, const CompositeArgs& a, const int* list,        \
-                               hipStream_t s);
+    void launch_composite_list(const Consts& k, const CompositeArgs& a, hipStream_t s);
 
 // Each namespace is reopened with the inline-ness this translation unit
 // gave it in atmo_device.h.
@@ -43,7 +42,7 @@ ATMO_REF_NAMESPACE {
 ATMO_DECLARE_LAUNCHERS
 // Reference precision only: the batched LUT builds.
 void launch_batch(const Consts& k, const BatchState* states, unsigned count, int width, int height,
-                  float* const luts[4], const long strides[4], hipStream_t s);
+                  float* const luts[4], const long strides[4], hipStream_t s, int g_t, int g_sv);
 }  // namespace ref
 ATMO_ROBUST_NAMESPACE {
 ATMO_DECLARE_LAUNCHERS

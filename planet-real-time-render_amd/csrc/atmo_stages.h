@@ -21,6 +21,11 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/atmo.h"
+
+// Waves per SIMD the COMPOSITE kernel is compiled for.
+#ifndef ATMO_COMPOSITE_WAVES
+#define ATMO_COMPOSITE_WAVES 4
+#endif
 #include "atmo_device.h"
 #include "atmo_handle.h"
 
@@ -464,71 +469,70 @@ __device__ __forceinline__ f3 pixel_ray(const CompositeArgs& a, int px, int py, 
     return normalized((a.right * sx + a.up * sy) + a.forward / a.thf);
 }
 
-// The COMPOSITE view of _composite (renderer.py:1860-1940) as two passes
-// over the same pixels: sky pixels (k_composite_sky) and surface pixels
-// (k_composite_surface).  Each pass walks 256-pixel groups with persistent
-// blocks and compacts its own pixel class onto the leading threads, so
-// waves run full and each pass gets the register budget of its own branch
-// (the surface branch alone fits three waves per SIMD).  Every pixel is
-// written by exactly one thread of one pass.
-__global__ void __launch_bounds__(256) k_composite_sky(Consts k, CompositeArgs a) {
-    __shared__ int list[256];
-    __shared__ int wave_counts[4];
-    const long npix = (long)a.width * a.height;
-    const long groups = (npix + 255) / 256;
-    for (long g = blockIdx.x; g < groups; g += gridDim.x) {
-        const long base = g * 256;
-        const long i = base + threadIdx.x;
-        const bool sky = i < npix && a.sid[i] < 0;
-        const int count = compact_block(sky, list, wave_counts);
-        if ((int)threadIdx.x < count) {
-            const long j = base + list[threadIdx.x];
-            f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height,
-                                a.right, a.up, a.forward, a.thf);
-            f3 c = composite_sky(k, a.tlut, a.svlut, ray, ld3(a.scene, j), ld3(a.space, j),
-                                 a.cam_alt, a.sky_alt, a.sun, a.disk, a.ar, a.thf, a.height, a.c0,
-                                 a.c1, a.bottom);
-            st3(a.hdr, j, maxs(c, 0.0f));
-        }
-        __syncthreads();  // `list` is rewritten by the next group
-    }
+// Sky branch of one COMPOSITE pixel (renderer.py:1882-1940).
+__device__ __forceinline__ void composite_sky_pixel(const Consts& k, const CompositeArgs& a,
+                                                    long i) {
+    f3 ray = camera_ray((int)(i / a.height), (int)(i % a.height), a.width, a.height, a.right,
+                        a.up, a.forward, a.thf);
+    f3 c = composite_sky(k, a.tlut, a.svlut, ray, ld3(a.scene, i), ld3(a.space, i), a.cam_alt,
+                         a.sky_alt, a.sun, a.disk, a.ar, a.thf, a.height, a.c0, a.c1, a.bottom);
+    st3(a.hdr, i, maxs(c, 0.0f));
 }
 
-// Three waves per SIMD (at most 168 VGPRs, no spills) measured 15 % faster
-// than the compiler's unconstrained two-wave allocation at 1080p.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
-k_composite_surface(Consts k, CompositeArgs a) {
+// The COMPOSITE view of _composite (renderer.py:1860-1940), one block per
+// 256 pixels.  Sky pixels are finished by their own thread; surface pixels
+// are then compacted onto the leading threads so waves run full, and run
+// the coarse-only limb integral.  A surface pixel with an interval whose
+// terminator-refinement weight is positive is deferred to k_composite_list,
+// which keeps the refinement's registers out of this kernel; every pixel is
+// written by exactly one thread.  Four waves per SIMD (128 VGPRs) is the
+// measured optimum at 1080p.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ATMO_COMPOSITE_WAVES, 8)))
+k_composite(Consts k, CompositeArgs a) {
     __shared__ int list[256];
     __shared__ int wave_counts[4];
     const long npix = (long)a.width * a.height;
-    const long groups = (npix + 255) / 256;
-    for (long g = blockIdx.x; g < groups; g += gridDim.x) {
-        const long base = g * 256;
-        const long i = base + threadIdx.x;
-        const bool surface = i < npix && a.sid[i] >= 0;
-        const int count = compact_block(surface, list, wave_counts);
-        if ((int)threadIdx.x < count) {
-            const long j = base + list[threadIdx.x];
-            f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height,
-                                a.right, a.up, a.forward, a.thf);
-            float sd = norm(ld3(a.pos, j));
-            f3 scat = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun,
-                                        a.irradiance, a.bottom, a.ar);
+    const long base = (long)blockIdx.x * 256;
+    const long i = base + threadIdx.x;
+    const bool live = i < npix;
+    const bool surface = live && a.sid[i] >= 0;
+    if (live && !surface) composite_sky_pixel(k, a, i);
+    const int count = compact_block(surface, list, wave_counts);
+    bool refine = false;
+    long j = 0;
+    if ((int)threadIdx.x < count) {
+        j = base + list[threadIdx.x];
+        f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height, a.right,
+                            a.up, a.forward, a.thf);
+        float sd = norm(ld3(a.pos, j));
+        f3 scat;
+        if (surface_inscatter_t<false>(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun, a.irradiance,
+                                       a.bottom, a.ar, scat)) {
             f3 trans = surface_transmittance(k, a.tlut, a.cam_alt, ray, sd, a.bottom);
             st3(a.hdr, j, maxs(scat + ld3(a.scene, j) * trans, 0.0f));
+        } else {
+            refine = true;
         }
-        __syncthreads();
     }
+    enqueue_wave(refine, j, a.queue, a.parity);
 }
 
-// Surface pixels listed in list[1 .. list[0]] (pixel indices): the plain
-// surface branch of the COMPOSITE view.  Used for the surrogate composite's
-// out-of-domain pixels.
-__global__ void __launch_bounds__(256) k_composite_list(Consts k, CompositeArgs a,
-                                                        const int* __restrict__ list) {
-    const int count = list[0];
+// Sky pixels only: the sky pass of the surrogate composite.
+__global__ void __launch_bounds__(256) k_composite_sky(Consts k, CompositeArgs a) {
+    const long npix = (long)a.width * a.height;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < npix && a.sid[i] < 0) composite_sky_pixel(k, a, i);
+}
+
+// The surface pixels deferred to the queue: the full surface branch of the
+// COMPOSITE view, refinement included (pixels the coarse pass could not
+// finish, or the surrogate composite's out-of-domain pixels).  Block 0 then
+// zeroes the other counter for the next call.
+__global__ void __launch_bounds__(256) k_composite_list(Consts k, CompositeArgs a) {
+    const int count = a.queue[a.parity];
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.queue[1 - a.parity] = 0;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < count; e += gridDim.x * blockDim.x) {
-        const long j = list[1 + e];
+        const long j = a.queue[2 + e];
         f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height, a.right,
                             a.up, a.forward, a.thf);
         float sd = norm(ld3(a.pos, j));
@@ -601,21 +605,26 @@ __global__ void __launch_bounds__(256) k_composite_diagnostic(Consts k, Composit
 
 // ------------------------------------------------------------ launchers
 
-constexpr int kGroupT = 8;   // lanes per transmittance texel
-constexpr int kGroupSV = 8;  // lanes per sky-view texel
-
 inline unsigned grid_for(long threads) { return (unsigned)((threads + 255) / 256); }
 
-// Persistent grid of the composite passes: enough 256-thread blocks for
-// several per CU on 256 CUs, never more than there are 256-pixel groups.
-inline unsigned composite_grid(long npix) {
-    const long groups = (npix + 255) / 256;
-    return (unsigned)(groups < 2048 ? groups : 2048);
+// Lanes per texel of the transmittance and sky-view builds: a multiple of
+// 2 that divides the wave (4, 8 or 16), chosen per handle or batch by the
+// autotuner (default 8).
+inline bool valid_group(int g) { return g == 4 || g == 8 || g == 16; }
+
+template <int G>
+void launch_transmittance_g(const Consts& k, float bottom, float* tlut, hipStream_t s) {
+    hipLaunchKernelGGL(k_transmittance<G>, dim3(grid_for((long)k.tw * k.th * G)), dim3(256), 0, s,
+                       k, bottom, tlut);
 }
 
-void launch_transmittance(const Consts& k, float bottom, float* tlut, hipStream_t s) {
-    hipLaunchKernelGGL(k_transmittance<kGroupT>, dim3(grid_for((long)k.tw * k.th * kGroupT)),
-                       dim3(256), 0, s, k, bottom, tlut);
+void launch_transmittance(const Consts& k, float bottom, float* tlut, hipStream_t s, int g) {
+    if (g == 4)
+        launch_transmittance_g<4>(k, bottom, tlut, s);
+    else if (g == 16)
+        launch_transmittance_g<16>(k, bottom, tlut, s);
+    else
+        launch_transmittance_g<8>(k, bottom, tlut, s);
 }
 
 void launch_multi_scattering(const Consts& k, float bottom, f3 irradiance, float ar,
@@ -624,12 +633,23 @@ void launch_multi_scattering(const Consts& k, float bottom, f3 irradiance, float
                        k, bottom, irradiance, ar, tlut, mslut);
 }
 
+template <int G>
+void launch_sky_view_g(const Consts& k, float bottom, float cam_alt, float sun_zc, f3 irradiance,
+                       float ar, const float* tlut, const float* mslut, float* svlut,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_sky_view<G>, dim3(grid_for((long)k.sw * k.sh * G)), dim3(256), 0, s, k,
+                       bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut, svlut);
+}
+
 void launch_sky_view(const Consts& k, float bottom, float cam_alt, float sun_zc, f3 irradiance,
-                     float ar, const float* tlut, const float* mslut, float* svlut,
-                     hipStream_t s) {
-    hipLaunchKernelGGL(k_sky_view<kGroupSV>, dim3(grid_for((long)k.sw * k.sh * kGroupSV)),
-                       dim3(256), 0, s, k, bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut,
-                       svlut);
+                     float ar, const float* tlut, const float* mslut, float* svlut, hipStream_t s,
+                     int g) {
+    if (g == 4)
+        launch_sky_view_g<4>(k, bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut, svlut, s);
+    else if (g == 16)
+        launch_sky_view_g<16>(k, bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut, svlut, s);
+    else
+        launch_sky_view_g<8>(k, bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut, svlut, s);
 }
 
 void launch_aerial(const Consts& k, int width, int height, float bottom, float cam_alt, f3 sun,
@@ -650,41 +670,61 @@ void launch_prepare(const Consts& k, long npix, const float* position_view,
 }
 
 void launch_composite_sky(const Consts& k, const CompositeArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_composite_sky, dim3(composite_grid((long)a.width * a.height)), dim3(256), 0,
-                       s, k, a);
+    hipLaunchKernelGGL(k_composite_sky, dim3(grid_for((long)a.width * a.height)), dim3(256), 0, s,
+                       k, a);
+}
+
+void launch_composite_list(const Consts& k, const CompositeArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_composite_list, dim3(256), dim3(256), 0, s, k, a);
 }
 
 void launch_composite(const Consts& k, const CompositeArgs& a, hipStream_t s) {
     const long npix = (long)a.width * a.height;
     if (a.view == ATMO_VIEW_COMPOSITE) {
-        hipLaunchKernelGGL(k_composite_sky, dim3(composite_grid(npix)), dim3(256), 0, s, k, a);
-        hipLaunchKernelGGL(k_composite_surface, dim3(composite_grid(npix)), dim3(256), 0, s, k, a);
+        hipLaunchKernelGGL(k_composite, dim3(grid_for(npix)), dim3(256), 0, s, k, a);
+        launch_composite_list(k, a, s);
     } else {
         hipLaunchKernelGGL(k_composite_diagnostic, dim3(grid_for(npix)), dim3(256), 0, s, k, a);
     }
 }
 
-void launch_composite_list(const Consts& k, const CompositeArgs& a, const int* list,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(k_composite_list, dim3(256), dim3(256), 0, s, k, a, list);
-}
-
 #if !ATMO_ROBUST
-void launch_batch(const Consts& k, const BatchState* states, unsigned count, int width, int height,
-                  float* const luts[4], const long strides[4], hipStream_t s) {
-    hipLaunchKernelGGL(k_transmittance_batched<kGroupT>,
-                       dim3(grid_for((long)k.tw * k.th * kGroupT), count), dim3(256), 0, s, states,
-                       luts[0], strides[0]);
+template <int GT, int GS>
+void launch_batch_g(const Consts& k, const BatchState* states, unsigned count, int width,
+                    int height, float* const luts[4], const long strides[4], hipStream_t s) {
+    hipLaunchKernelGGL(k_transmittance_batched<GT>, dim3(grid_for((long)k.tw * k.th * GT), count),
+                       dim3(256), 0, s, states, luts[0], strides[0]);
     hipLaunchKernelGGL(k_multi_scattering_batched, dim3(grid_for((long)k.mw * k.mh * 64), count),
                        dim3(256), 0, s, states, (const float*)luts[0], strides[0], luts[1],
                        strides[1]);
-    hipLaunchKernelGGL(k_sky_view_batched<kGroupSV>,
-                       dim3(grid_for((long)k.sw * k.sh * kGroupSV), count), dim3(256), 0, s, states,
-                       (const float*)luts[0], strides[0], (const float*)luts[1], strides[1],
-                       luts[2], strides[2]);
+    hipLaunchKernelGGL(k_sky_view_batched<GS>, dim3(grid_for((long)k.sw * k.sh * GS), count),
+                       dim3(256), 0, s, states, (const float*)luts[0], strides[0],
+                       (const float*)luts[1], strides[1], luts[2], strides[2]);
     hipLaunchKernelGGL(k_aerial_batched, dim3(grid_for((long)k.aw * k.ah * 64), count), dim3(256), 0,
                        s, states, width, height, (const float*)luts[0], strides[0],
                        (const float*)luts[1], strides[1], luts[3], strides[3]);
+}
+
+template <int GT>
+void launch_batch_gt(const Consts& k, const BatchState* states, unsigned count, int width,
+                     int height, float* const luts[4], const long strides[4], hipStream_t s,
+                     int g_sv) {
+    if (g_sv == 4)
+        launch_batch_g<GT, 4>(k, states, count, width, height, luts, strides, s);
+    else if (g_sv == 16)
+        launch_batch_g<GT, 16>(k, states, count, width, height, luts, strides, s);
+    else
+        launch_batch_g<GT, 8>(k, states, count, width, height, luts, strides, s);
+}
+
+void launch_batch(const Consts& k, const BatchState* states, unsigned count, int width, int height,
+                  float* const luts[4], const long strides[4], hipStream_t s, int g_t, int g_sv) {
+    if (g_t == 4)
+        launch_batch_gt<4>(k, states, count, width, height, luts, strides, s, g_sv);
+    else if (g_t == 16)
+        launch_batch_gt<16>(k, states, count, width, height, luts, strides, s, g_sv);
+    else
+        launch_batch_gt<8>(k, states, count, width, height, luts, strides, s, g_sv);
 }
 #endif
 

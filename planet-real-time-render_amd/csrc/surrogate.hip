@@ -506,10 +506,10 @@ struct SurrogateArgs {
 // compacted onto the leading threads, so the network runs in full waves
 // over surface pixels only (the MFMA chain needs the whole wave; a wave runs
 // it when any lane holds an in-domain pixel).  Out-of-domain surface pixels
-// are queued for k_composite_fallback, which keeps the plain integrator's
+// are queued for k_composite_list, which keeps the plain integrator's
 // registers out of this kernel.
 __global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, SurrogateArgs a,
-                                                             int* __restrict__ queue) {
+                                                             int* __restrict__ queue, int parity) {
     __shared__ bf16x4 frag_s[kFragments * 64];
     __shared__ float bias_s[64 * 3 + 16];
     __shared__ int list[256];
@@ -536,14 +536,8 @@ __global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, Surrogate
                                   a.right, a.up, a.forward, a.thf);
         const float sd = has ? norm(ld3(a.pos, j)) : 1.0f;
         const bool learned = has && in_domain(a.t, a.cam_alt, sd);
-        // queue the out-of-domain surface pixels (one atomic per wave)
-        const unsigned long long fb = __ballot(has && !learned);
-        if (fb) {
-            int slot = 0;
-            if (lane == 0) slot = atomicAdd(queue, __popcll(fb));
-            slot = __shfl(slot, 0, 64);
-            if (has && !learned) queue[1 + slot + __popcll(fb & ((1ull << lane) - 1ull))] = (int)j;
-        }
+        // queue the out-of-domain surface pixels
+        enqueue_wave(has && !learned, j, queue, parity);
         if (__ballot(learned)) {  // wave-uniform branch
             float x[5];
             // relative azimuth as in the sky-view sampler (renderer.py:1482-1497)
@@ -568,7 +562,7 @@ __global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, Surrogate
     }
 }
 
-__global__ void k_zero_queue(int* __restrict__ queue) { queue[0] = 0; }
+
 
 // Batch inference on encoded features (testing and evaluation): x [n, 5]
 // -> y [n, 6] raw network outputs, through the same MFMA path.
@@ -911,12 +905,14 @@ int32_t atmo_composite_surrogate(AtmoHandle* h, AtmoMlp* m, const float* scene_h
     ca.c0 = c0;
     ca.c1 = c1;
     ca.view = ATMO_VIEW_COMPOSITE;
+    ca.queue = h->queue;
+    ca.parity = h->queue_parity;
+    h->queue_parity ^= 1;
     if (atmo_launch_composite_sky(h, ca, s) != ATMO_OK) return ATMO_EHIP;
-    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, s, h->queue);
     hipLaunchKernelGGL(k_composite_surrogate, dim3((unsigned)(groups < 1024 ? groups : 1024)),
-                       dim3(256), 0, s, h->k, a, h->queue);
+                       dim3(256), 0, s, h->k, a, h->queue, ca.parity);
     if (atmo_launch_status(h, "atmo_composite_surrogate") != ATMO_OK) return ATMO_EHIP;
-    return atmo_launch_composite_list(h, ca, h->queue, s);
+    return atmo_launch_composite_list(h, ca, s);
 }
 
 }  // extern "C"
