@@ -45,9 +45,14 @@ COST = {
     "sky_pixel": (328.0, 32.0),
     "refined_interval": (2946.0, 336.0),
 }
-# Composite bytes per pixel: scene, space, position (12 B each) + id (4 B)
-# read, hdr (12 B) written -- every pixel reads all inputs in the reference.
-COMPOSITE_BYTES_PER_PIXEL = 12 + 12 + 12 + 4 + 12
+# Composite algorithmic bytes per pixel: surface_id (4 B) and two 12-byte
+# inputs read (scene + space for a sky pixel, scene + position for a surface
+# pixel), hdr (12 B) written.
+COMPOSITE_BYTES_PER_PIXEL = 4 + 12 + 12 + 12
+# HBM bytes one 1080p k_composite launch moves, from the FETCH_SIZE and
+# WRITE_SIZE counter passes (scaled by a 256 MiB calibration copy) recorded
+# in profiles/atmosphere_frame_1080p_hbm_traffic.md.
+COMPOSITE_TRAFFIC_BYTES_1080P = 127.19e6
 
 
 def log(*args):
@@ -379,8 +384,11 @@ def main() -> int:
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4),
-        "traffic": None,
-        "kernel": "k_composite_sky + k_composite_surface (atmo_composite)",
+        "traffic": (COMPOSITE_TRAFFIC_BYTES_1080P
+                    if (inputs.width, inputs.height) == (1920, 1080) else None),
+        "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
+        "kernel": "k_composite + k_composite_list (atmo_composite, COMPOSITE view)",
+        "algorithmic_bytes": bytes_per_launch,
         "kernel_ms": round(stage_ms["composite"], 4),
         "algorithmic_flop": flops,
         "algorithmic_transcendental": tops,

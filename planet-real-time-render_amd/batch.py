@@ -98,6 +98,20 @@ class LutBatch:
             raise _capi.AtmoError(self._lib.atmo_batch_last_error(self._handle)
                                   .decode(errors="replace"))
 
+    def autotune(self, stream=None) -> tuple[int, int]:
+        """Time the nine (lanes per T texel, lanes per sky-view texel)
+        launch shapes of the batch's stages and keep the fastest for the
+        captured graph (atmo_batch_autotune).  Synchronises the stream."""
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        chosen = (ctypes.c_int32 * 2)()
+        status = self._lib.atmo_batch_autotune(self._handle, s.cuda_stream, chosen)
+        if status != _capi.ATMO_OK:
+            raise _capi.AtmoError(self._lib.atmo_batch_last_error(self._handle)
+                                  .decode(errors="replace"))
+        return int(chosen[0]), int(chosen[1])
+
     @property
     def transmittance(self):
         return self.luts[0]

@@ -122,16 +122,39 @@ __device__ __forceinline__ void transmittance_body(const Consts& k, float bottom
     if (active && lane == 0) st3(tlut, texel, clamp3(exp3(-mins(od, 80.0f)), 0.0f, 1.0f));
 }
 
-// _build_multi_scattering (renderer.py:1065-1206).  One wave64 per texel,
-// lane = Fibonacci-sphere direction.
+// Ordered combination of consecutive (L, F, T) segments of one
+// multi-scattering direction within a group of G lanes (see scan_segments):
+// afterwards lane i holds segments 0..i.
+template <int G>
+__device__ __forceinline__ void scan_ms_segments(f3& L, f3& F, f3& T, int lane) {
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+        f3 pl = mk(__shfl_up(L.x, off, G), __shfl_up(L.y, off, G), __shfl_up(L.z, off, G));
+        f3 pf = mk(__shfl_up(F.x, off, G), __shfl_up(F.y, off, G), __shfl_up(F.z, off, G));
+        f3 pt = mk(__shfl_up(T.x, off, G), __shfl_up(T.y, off, G), __shfl_up(T.z, off, G));
+        if (lane >= off) {
+            L = pl + pt * L;
+            F = pf + pt * F;
+            T = pt * T;
+        }
+    }
+}
+
+// _build_multi_scattering (renderer.py:1065-1206).  One 256-thread block per
+// texel: 64 Fibonacci-sphere directions at a time, kMsLanes lanes per
+// direction, each lane marching a contiguous chunk of the direction's steps;
+// the chunks combine by an ordered (L, F, T) scan, the group's last lane adds
+// the ground bounce with the direction's full transmittance, and the
+// direction sums reduce across the block.
+constexpr int kMsLanes = 4;
+
 __device__ __forceinline__ void multi_scattering_body(const Consts& k, float bottom, f3 irradiance,
                                                       float ar, const float* __restrict__ tlut,
-                                                      float* __restrict__ mslut, long gid) {
-    const long wave = gid / 64;
-    const int lane = threadIdx.x % 64;
-    const long n = (long)k.mw * k.mh;
-    if (wave >= n) return;  // uniform per wave
-    const int x = (int)(wave / k.mh), y = (int)(wave % k.mh);
+                                                      float* __restrict__ mslut, long texel) {
+    __shared__ float partial[4][6];
+    const int tid = threadIdx.x;
+    const int chunk = tid % kMsLanes;
+    const int x = (int)(texel / k.mh), y = (int)(texel % k.mh);
     float u = ((float)x + 0.5f) / (float)k.mw;
     float v = ((float)y + 0.5f) / (float)k.mh;
     float h0 = v * v * k.top;
@@ -142,37 +165,47 @@ __device__ __forceinline__ void multi_scattering_body(const Consts& k, float bot
     f3 origin = mk(0.0f, radius, 0.0f);
     const int N = k.ms_dirs;
     const int S = k.ms_steps;
+    const int per = (S + kMsLanes - 1) / kMsLanes;
+    const int s0 = min(chunk * per, S), s1 = min(s0 + per, S);
     f3 lum = mk(0.0f, 0.0f, 0.0f), fb = mk(0.0f, 0.0f, 0.0f);
-    for (int di = lane; di < N; di += 64) {
-        float si = (float)di + 0.5f;
-        float dc = 1.0f - 2.0f * si / (float)N;
-        float ds = sqrtf(fmaxf(1.0f - dc * dc, 0.0f));
-        float az = (float)di * 2.39996323f;
-        f3 ray = mk(ds * cr_cos(az), dc, ds * cr_sin(az));
-        float start, end;
-        bool valid, ground;
-        segment_vec(k, origin, ray, bottom, start, end, valid, ground);
+    for (int base = 0; base < N; base += 256 / kMsLanes) {
+        const int di = base + tid / kMsLanes;
         f3 dl = mk(0.0f, 0.0f, 0.0f), df = mk(0.0f, 0.0f, 0.0f), pt = mk(1.0f, 1.0f, 1.0f);
-        if (valid) {
-            Warp w = make_warp(-dot(origin, ray), start, end);
-            RayPhase ph = ray_phase(k, ray, sun);
-            float d0 = boundary(w, 0.0f / (float)S);
-            for (int s = 0; s < S; ++s) {
-                float d1 = boundary(w, (float)(s + 1) / (float)S);
-                float dist = (d0 + d1) * 0.5f;
-                float step = d1 - d0;
-                f3 point = origin + ray * dist;
-                float h = altitude_on_ray(h0, dc, dist, bottom);
-                f3 d = density(k, h);
-                f3 st, si3;
-                segment_integral(extinction_of(k, d), step, st, si3);
-                f3 direct = scattering_source<false>(k, tlut, nullptr, point, h, d, ph, sun,
-                                                     irradiance, bottom, ar);
-                dl = dl + (pt * direct) * si3;
-                df = df + (pt * scattering_of(k, d)) * si3;
-                pt = pt * st;
-                d0 = d1;
+        bool valid = false, ground = false;
+        float end = 0.0f;
+        f3 ray = mk(0.0f, 1.0f, 0.0f);
+        if (di < N) {
+            float si = (float)di + 0.5f;
+            float dc = 1.0f - 2.0f * si / (float)N;
+            float ds = sqrtf(fmaxf(1.0f - dc * dc, 0.0f));
+            float az = (float)di * 2.39996323f;
+            ray = mk(ds * cr_cos(az), dc, ds * cr_sin(az));
+            float start;
+            segment_vec(k, origin, ray, bottom, start, end, valid, ground);
+            if (valid && s0 < s1) {
+                Warp w = make_warp(-dot(origin, ray), start, end);
+                RayPhase ph = ray_phase(k, ray, sun);
+                float d0 = boundary(w, (float)s0 / (float)S);
+                for (int s = s0; s < s1; ++s) {
+                    float d1 = boundary(w, (float)(s + 1) / (float)S);
+                    float dist = (d0 + d1) * 0.5f;
+                    float step = d1 - d0;
+                    f3 point = origin + ray * dist;
+                    float h = altitude_on_ray(h0, dc, dist, bottom);
+                    f3 d = density(k, h);
+                    f3 st, si3;
+                    segment_integral(extinction_of(k, d), step, st, si3);
+                    f3 direct = scattering_source<false>(k, tlut, nullptr, point, h, d, ph, sun,
+                                                         irradiance, bottom, ar);
+                    dl = dl + (pt * direct) * si3;
+                    df = df + (pt * scattering_of(k, d)) * si3;
+                    pt = pt * st;
+                    d0 = d1;
+                }
             }
+        }
+        scan_ms_segments<kMsLanes>(dl, df, pt, chunk);
+        if (chunk == kMsLanes - 1 && di < N && valid) {
             if (ground) {
                 f3 gp = origin + ray * end;
                 f3 gn = normalized(gp);
@@ -182,9 +215,9 @@ __device__ __forceinline__ void multi_scattering_body(const Consts& k, float bot
                 f3 gl = (((arr3(k.albedo) * irradiance) * gt) * fmaxf(gsc, 0.0f)) / k.pi;
                 dl = dl + pt * gl;
             }
+            lum = lum + dl;
+            fb = fb + df;
         }
-        lum = lum + dl;
-        fb = fb + df;
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -195,10 +228,25 @@ __device__ __forceinline__ void multi_scattering_body(const Consts& k, float bot
         fb.y += __shfl_xor(fb.y, off, 64);
         fb.z += __shfl_xor(fb.z, off, 64);
     }
-    if (lane == 0) {
-        f3 avg_l = lum / (float)N;
-        f3 avg_f = clamp3(fb / (float)N, 0.0f, 0.95f);
-        st3(mslut, wave, maxs(avg_l / (mk(1.0f, 1.0f, 1.0f) - avg_f), 0.0f));
+    const int wave = tid / 64;
+    if (tid % 64 == 0) {
+        partial[wave][0] = lum.x;
+        partial[wave][1] = lum.y;
+        partial[wave][2] = lum.z;
+        partial[wave][3] = fb.x;
+        partial[wave][4] = fb.y;
+        partial[wave][5] = fb.z;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        f3 sl = mk(0.0f, 0.0f, 0.0f), sf = mk(0.0f, 0.0f, 0.0f);
+        for (int w = 0; w < 4; ++w) {
+            sl = sl + mk(partial[w][0], partial[w][1], partial[w][2]);
+            sf = sf + mk(partial[w][3], partial[w][4], partial[w][5]);
+        }
+        f3 avg_l = sl / (float)N;
+        f3 avg_f = clamp3(sf / (float)N, 0.0f, 0.95f);
+        st3(mslut, texel, maxs(avg_l / (mk(1.0f, 1.0f, 1.0f) - avg_f), 0.0f));
     }
 }
 
@@ -339,8 +387,7 @@ __global__ void __launch_bounds__(256) k_transmittance(Consts k, float bottom, f
 __global__ void __launch_bounds__(256) k_multi_scattering(Consts k, float bottom, f3 irradiance,
                                                           float ar, const float* __restrict__ tlut,
                                                           float* __restrict__ mslut) {
-    multi_scattering_body(k, bottom, irradiance, ar, tlut, mslut,
-                          (long)blockIdx.x * blockDim.x + threadIdx.x);
+    multi_scattering_body(k, bottom, irradiance, ar, tlut, mslut, (long)blockIdx.x);
 }
 
 template <int G>
@@ -384,7 +431,7 @@ __global__ void __launch_bounds__(256) k_multi_scattering_batched(const BatchSta
                                                                   long m_stride) {
     const BatchState& b = st[blockIdx.y];
     multi_scattering_body(b.k, b.bottom, b.irradiance, b.ar_update, tluts + blockIdx.y * t_stride,
-                          msluts + blockIdx.y * m_stride, (long)blockIdx.x * blockDim.x + threadIdx.x);
+                          msluts + blockIdx.y * m_stride, (long)blockIdx.x);
 }
 
 template <int G>
@@ -629,8 +676,8 @@ void launch_transmittance(const Consts& k, float bottom, float* tlut, hipStream_
 
 void launch_multi_scattering(const Consts& k, float bottom, f3 irradiance, float ar,
                              const float* tlut, float* mslut, hipStream_t s) {
-    hipLaunchKernelGGL(k_multi_scattering, dim3(grid_for((long)k.mw * k.mh * 64)), dim3(256), 0, s,
-                       k, bottom, irradiance, ar, tlut, mslut);
+    hipLaunchKernelGGL(k_multi_scattering, dim3((unsigned)(k.mw * k.mh)), dim3(256), 0, s, k, bottom,
+                       irradiance, ar, tlut, mslut);
 }
 
 template <int G>
@@ -694,7 +741,7 @@ void launch_batch_g(const Consts& k, const BatchState* states, unsigned count, i
                     int height, float* const luts[4], const long strides[4], hipStream_t s) {
     hipLaunchKernelGGL(k_transmittance_batched<GT>, dim3(grid_for((long)k.tw * k.th * GT), count),
                        dim3(256), 0, s, states, luts[0], strides[0]);
-    hipLaunchKernelGGL(k_multi_scattering_batched, dim3(grid_for((long)k.mw * k.mh * 64), count),
+    hipLaunchKernelGGL(k_multi_scattering_batched, dim3((unsigned)(k.mw * k.mh), count),
                        dim3(256), 0, s, states, (const float*)luts[0], strides[0], luts[1],
                        strides[1]);
     hipLaunchKernelGGL(k_sky_view_batched<GS>, dim3(grid_for((long)k.sw * k.sh * GS), count),

@@ -242,6 +242,45 @@ class AtmosphereRenderer:
             sun_angular_radius, f3(right), f3(view_up), f3(forward), tangent_half_fov,
             star_contrast_start, star_contrast_end, self._stream()), self._handle)
 
+    # ------------------------------------------------------------ autotuner
+
+    def autotune(self, planet_radius_m: float, camera_radius_m: float, sun_local,
+                 solar_irradiance, sun_angular_radius_degrees: float) -> tuple[int, int]:
+        """Time the transmittance and sky-view launch shapes (lanes per
+        texel 4, 8, 16) with this renderer's LUT sizes and keep the fastest
+        (atmo_autotune).  Arguments as for `update`; the LUTs are rebuilt,
+        so every cache key is reset.  Returns (lanes per T texel, lanes per
+        sky-view texel)."""
+        planet_radius = float(planet_radius_m)
+        sun = np.asarray(sun_local, dtype=np.float64)
+        sun = sun / max(float(np.linalg.norm(sun)), 1.0e-12)
+        chosen = (ctypes.c_int32 * 2)()
+        check(self._lib.atmo_autotune(
+            self._handle, planet_radius, f3(solar_irradiance),
+            float(np.float32(math.radians(sun_angular_radius_degrees))),
+            float(np.float32(max(float(camera_radius_m) - planet_radius, 0.0))),
+            float(np.float32(np.clip(sun[1], -1.0, 1.0))), self._stream(), chosen), self._handle)
+        self._planet_radius_key = None
+        self._multi_scattering_key = None
+        self._sky_key = None
+        return int(chosen[0]), int(chosen[1])
+
+    @property
+    def tuning(self) -> tuple[int, int]:
+        t = (ctypes.c_int32 * 2)()
+        check(self._lib.atmo_get_tuning(self._handle, t), self._handle)
+        return int(t[0]), int(t[1])
+
+    @tuning.setter
+    def tuning(self, value) -> None:
+        """Set (lanes per T texel, lanes per sky-view texel); the next
+        `update` rebuilds every LUT with the new shapes."""
+        t = (ctypes.c_int32 * 2)(*[int(v) for v in value])
+        check(self._lib.atmo_set_tuning(self._handle, t), self._handle)
+        self._planet_radius_key = None
+        self._multi_scattering_key = None
+        self._sky_key = None
+
     # ----------------------------------------------------------- host policy
 
     def set_luts_frozen(self, frozen: bool) -> None:

@@ -149,3 +149,29 @@ def test_batch_rejects_mismatched_lut_sizes(scene, batch_mod, device):
     states[1] = OutOfRange(states[1])
     with pytest.raises(ValueError, match="state 1: atmosphere.mie_phase_g"):
         batch_mod.LutBatch(states, WIDTH, HEIGHT, device)
+
+
+def test_batch_autotune_keeps_parity(scene, batch_mod, ref32, device):
+    """The autotuned launch shapes only reassociate sums: a sample of states
+    stays within the parity bar of the oracle, and graph replays are
+    stable after the re-capture."""
+    import torch
+
+    states = scene.batch_states(16)
+    b = batch_mod.LutBatch(states, WIDTH, HEIGHT, device)
+    chosen = b.autotune()
+    print(f"batch autotune (16 states): lanes per T texel {chosen[0]}, per SV texel {chosen[1]}",
+          flush=True)
+    assert chosen[0] in (4, 8, 16) and chosen[1] in (4, 8, 16)
+    b.run()
+    torch.cuda.synchronize(device)
+    first = [t.clone() for t in b.luts]
+    b.run()
+    torch.cuda.synchronize(device)
+    for t, f in zip(b.luts, first):
+        assert torch.equal(t, f)
+    s = states[3]
+    t = ref32.build_transmittance(s.config, float(s.planet_radius_m))
+    e = summary(rel_error(b.transmittance[3].cpu().numpy(), t))
+    assert e["max"] <= PLAIN_TOLERANCE
+    b.close()

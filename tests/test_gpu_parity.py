@@ -249,3 +249,37 @@ def test_argument_errors(pkg, scene, device):
     with pytest.raises(ValueError):
         pkg.AtmosphereRenderer(16, 8, inputs.config.replace(sky_view_steps=2), device)
     r.close()
+
+
+def test_autotuned_shapes_keep_parity(pkg, scene, ref32, device):
+    """Every (lanes per T texel, lanes per SV texel) shape the autotuner may
+    choose stays within the parity bar; the autotuner picks one of them and
+    leaves T, MS and SV built."""
+    cfg = scene.full_frame_config()
+    R, r_cam, sun = scene.lut_chain_state(0)
+    E = scene.SOLAR_IRRADIANCE
+    r = pkg.AtmosphereRenderer(64, 36, cfg, device)
+    chosen = r.autotune(R, r_cam, sun, E, scene.SUN_ANGULAR_RADIUS_DEG)
+    print(f"autotune: lanes per T texel {chosen[0]}, per SV texel {chosen[1]}", flush=True)
+    assert chosen in [(gt, gs) for gt in (4, 8, 16) for gs in (4, 8, 16)]
+    assert r.tuning == chosen
+    ar = float(np.float32(math.radians(scene.SUN_ANGULAR_RADIUS_DEG)))
+    t = ref32.build_transmittance(cfg, R)
+    ms = ref32.build_multi_scattering(cfg, R, E, ar, t)
+    sun_n = np.asarray(sun) / np.linalg.norm(sun)
+    sv = ref32.build_sky_view(cfg, R, float(np.float32(r_cam - R)), float(np.float32(sun_n[1])),
+                              E, ar, t, ms)
+    for g in (4, 8, 16):
+        r.tuning = (g, g)
+        r.update(R, r_cam, sun, E, scene.SUN_ANGULAR_RADIUS_DEG)
+        import torch
+
+        torch.cuda.synchronize(device)
+        for name, got, want in (("T", r.transmittance_lut.to_numpy(), t),
+                                ("MS", r.multi_scattering_lut.to_numpy(), ms),
+                                ("SV", r.sky_view_lut.to_numpy(), sv)):
+            e = summary(rel_error(got, want))
+            assert e["max"] <= PLAIN_TOLERANCE, (g, name, e["max"])
+    with pytest.raises(ValueError, match="4, 8 or 16"):
+        r.tuning = (3, 8)
+    r.close()

@@ -237,3 +237,60 @@ def test_surrogate_composite_falls_back_outside_domain(pkg, scene, sur, device):
         np.testing.assert_array_equal(outs[0], outs[1])
         r.close()
     net.close()
+
+
+def test_surrogate_gain_against_fewer_integrator_steps(pkg, scene, sur, ref64, device):
+    """What the trained network buys: the 1080p COMPOSITE with the shipped
+    surrogate against the plain integrator at the reference's 12 base
+    intervals and at fewer (4, 6, 8), each timed on the GPU and scored by
+    its p99 relative error against the float64 restatement (12 intervals).
+    The surrogate must be faster than the 12-interval path; the equal-time
+    accuracy comparison is printed for the record."""
+    import torch
+
+    weights = sur.default_weights_path()
+    net = sur.SurrogateMLP.load(weights, max_batch=1024, device=device)
+    inputs = scene.make_frame(width=1920, height=1080, config_index=1, seed_offset=0)
+    want = ref64.frame(inputs)["hdr"]
+    light = inputs.lighting
+
+    def run(config, surrogate, reps=10):
+        r = pkg.AtmosphereRenderer(inputs.width, inputs.height, config, device)
+        pos = torch.from_numpy(inputs.position_view).to(device)
+        sid = torch.from_numpy(inputs.surface_id).to(device)
+        sc = torch.from_numpy(inputs.scene_hdr).to(device)
+        sp = torch.from_numpy(inputs.space_hdr).to(device)
+        hdr = torch.zeros_like(sc)
+        r.update(inputs.planet_radius_m, inputs.camera_radius_m, inputs.sun_local,
+                 light.solar_irradiance, light.sun_angular_radius_degrees)
+        r.prepare_frame(pos, sid, inputs.planet_radius_m, inputs.camera_radius_m,
+                        inputs.sun_local, light.solar_irradiance,
+                        light.sun_angular_radius_degrees, inputs.view_basis,
+                        inputs.tangent_half_fov)
+        call = lambda: r.composite(sc, sp, sid, pos, hdr, inputs.planet_radius_m,
+                                   inputs.camera_radius_m, inputs.sun_local,
+                                   light.solar_irradiance, light.sun_disk_radiance,
+                                   light.sun_angular_radius_degrees, inputs.view_basis,
+                                   inputs.tangent_half_fov, *inputs.star_contrast,
+                                   surrogate=surrogate)
+        call()
+        torch.cuda.synchronize(device)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            call()
+        e1.record()
+        torch.cuda.synchronize(device)
+        p99 = summary(rel_error(hdr.cpu().numpy(), want))["p99"]
+        r.close()
+        return e0.elapsed_time(e1) / reps, p99
+
+    base = inputs.config
+    results = {"surrogate": run(base, net)}
+    for steps in (12, 8, 6, 4):
+        cfg = base.replace(aerial_horizon_raymarch_steps=steps)
+        results[f"plain {steps} intervals"] = run(cfg, None)
+    for name, (ms, p99) in results.items():
+        print(f"gain: {name:20s} composite {ms * 1e3:8.1f} us  p99 vs f64 {p99:.2e}", flush=True)
+    assert results["surrogate"][0] < results["plain 12 intervals"][0]
+    net.close()
