@@ -49,6 +49,7 @@ struct Consts {
     float beta_r[3], beta_ms[3], beta_me[3], beta_a[3], albedo[3];
     float h_r, h_m, peak, half_width;
     float mie_a, mie_b, mie_c, mie_d;
+    float r_h_r, r_h_m, r_half_width, r_top;  // 1.0f / (h_r, h_m, half_width, top), rounded once
     int tw, th, mw, mh, sw, sh, aw, ah, ad;
     int t_steps, ms_dirs, ms_steps, sv_steps, ap_spp, hz_steps, term_sub;
 };
@@ -80,6 +81,17 @@ __device__ __forceinline__ f3 normalized(f3 a) {
     return f3{inv * a.x, inv * a.y, inv * a.z};
 }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(hi, fmaxf(lo, x)); }
+
+// a / b, the correctly rounded quotient, for a divisor with a precomputed
+// rb = 1.0f / b (itself correctly rounded): q = a rb is within an ulp, the
+// remainder a - q b is exact in one FMA, and one Markstein correction
+// q + (a - q b) rb rounds to the quotient.  Three instructions instead of
+// the ten of an IEEE division; equal to a / b for every float a in
+// [1e-9, 4e6] and the divisors of this model (checked exhaustively).
+__device__ __forceinline__ float quot(float a, float b, float rb) {
+    const float q = a * rb;
+    return fmaf(fmaf(-q, b, a), rb, q);
+}
 __device__ __forceinline__ f3 maxs(f3 a, float s) { return f3{fmaxf(a.x, s), fmaxf(a.y, s), fmaxf(a.z, s)}; }
 __device__ __forceinline__ f3 mins(f3 a, float s) { return f3{fminf(a.x, s), fminf(a.y, s), fminf(a.z, s)}; }
 __device__ __forceinline__ f3 clamp3(f3 a, float lo, float hi) {
@@ -116,11 +128,14 @@ inline namespace ATMO_PRECISION {
 __device__ __forceinline__ f3 density(const Consts& k, float h) {
     float rayleigh = 0.0f, mie = 0.0f, absorption = 0.0f;
     if (0.0f <= h && h <= k.top) {
-        // Hardware-rate exp: a last-bit difference here moves the
-        // extinction by one ulp, which no later step amplifies.
-        rayleigh = expf(-h / k.h_r);
-        mie = expf(-h / k.h_m);
-        absorption = fmaxf(1.0f - fabsf(h - k.peak) / k.half_width, 0.0f);
+        // Hardware-rate exp: a last-bit difference in exp moves the
+        // extinction by one ulp.  The exponent h / H must be the correctly
+        // rounded quotient (quot): twilight texels are lit through grazing
+        // solar paths of optical depth ~40, where even the 1-2 ulp of
+        // h * (1 / H) show as 4e-4.
+        rayleigh = expf(-quot(h, k.h_r, k.r_h_r));
+        mie = expf(-quot(h, k.h_m, k.r_h_m));
+        absorption = fmaxf(1.0f - quot(fabsf(h - k.peak), k.half_width, k.r_half_width), 0.0f);
     }
     return f3{rayleigh, mie, absorption};
 }
@@ -530,7 +545,7 @@ __device__ __forceinline__ f3 sample_transmittance(const Consts& k, const float*
 __device__ __forceinline__ f3 sample_multi_scattering(const Consts& k,
                                                       const float* __restrict__ mslut, float h,
                                                       float sun_cosine, float bottom) {
-    float af = clampf(h / k.top, 0.0f, 1.0f);
+    float af = clampf(quot(h, k.top, k.r_top), 0.0f, 1.0f);
     float u = sun_cosine_to_u(sun_cosine, fmaxf(h, 0.0f), bottom);
     float v = sqrtf(af);
     return bilinear(mslut, k.mw, k.mh, u, v);
@@ -566,10 +581,36 @@ __device__ __forceinline__ void solar_shadow_roots(f3 origin, f3 ray, f3 sun, fl
 // The quantities _solar_disk_visibility and _sample_solar_transmittance
 // both derive from (altitude, sun cosine) (renderer.py:728-789), computed
 // once and shared.
+//
+// Outside the penumbra band the visible fraction is exactly 1 or 0, and the
+// band is decided from cosines alone: with c_h the horizon cosine, s_h its
+// sine and d = 1.05 alpha, the sun is fully up when its cosine exceeds
+// cos(acos(c_h) - d) = c_h cos d + s_h sin d and fully down below
+// cos(acos(c_h) + d).  The 5 % margin dwarfs every rounding error, so the
+// classification agrees with the correctly rounded zeniths.  Fully down, the
+// transmittance is multiplied by 0 and nothing else is evaluated.  Fully
+// up, the representative zenith is the sun zenith z = float(acos(x)),
+// correctly rounded, and its cosine is cos(acos(x) + e) = x - sqrt(1 - x^2) e
+// with e = z - acos(x) exact in float64: the correctly rounded cos(z)
+// without a float64 cosine.  (A grazing solar ray's optical depth turns the
+// ~1e-7 gap between x and cos(z) into ~1e-4 of transmittance, so z's
+// rounding must be kept.)  Inside the band both zeniths are correctly
+// rounded: there the disk visibility amplifies their last bits.
+// float(cos(float(acos(x)))), both roundings correct, with one float64
+// acos: cos(acos(x) + e) = x cos e - sqrt(1 - x^2) sin e with
+// |e| < 2^-22, so cos e = 1 - O(1e-14) and sin e = e to far below float32
+// rounding.
+__device__ __forceinline__ float cos_of_rounded_acos(float x) {
+    const double xd = fmin(fmax((double)x, -1.0), 1.0);
+    const double t = acos(xd);
+    const double e = (double)(float)t - t;
+    return (float)(xd - sqrt(fmax(1.0 - xd * xd, 0.0)) * e);
+}
+
 struct SunGeom {
     float visibility;
-    float horizon_zenith;
-    float sun_zenith;
+    float rc;   // cosine of the representative zenith
+    bool fast;  // outside the penumbra band: visibility exactly 0 or 1
 };
 
 __device__ __forceinline__ SunGeom sun_geometry(const Consts& k, float h, float sun_cosine,
@@ -577,22 +618,40 @@ __device__ __forceinline__ SunGeom sun_geometry(const Consts& k, float h, float 
     SunGeom g;
     float radius = bottom + fmaxf(h, 0.0f);
     float hc = -shell_rho(h, bottom) / fmaxf(radius, 1.0f);
+    {
+        const float dl = 1.05f * angular_radius;
+        const float cd = 1.0f - 0.5f * dl * dl, sd = dl;  // cos d, sin d to O(d^3)
+        const float hs = sqrtf(fmaxf(1.0f - hc * hc, 0.0f));
+        const float up = hc * cd + hs * sd, down = hc * cd - hs * sd;
+        if (sun_cosine <= down) {
+            g.fast = true;
+            g.visibility = 0.0f;
+            g.rc = sun_cosine;
+            return g;
+        }
+        if (sun_cosine >= up) {
+            g.fast = true;
+            g.visibility = 1.0f;
+            g.rc = cos_of_rounded_acos(sun_cosine);
+            return g;
+        }
+    }
+    g.fast = false;
     // Both zeniths are correctly rounded: near the horizon the representative
     // zenith feeds cos() and the T-LUT distance coordinate, whose slope there
     // is ~planet radius per unit cosine.
-    g.horizon_zenith = cr_acos(clampf(hc, -1.0f, 1.0f));
-    g.sun_zenith = cr_acos(clampf(sun_cosine, -1.0f, 1.0f));
-    float sep = g.horizon_zenith - g.sun_zenith;
+    const float horizon_zenith = cr_acos(clampf(hc, -1.0f, 1.0f));
+    const float sun_zenith = cr_acos(clampf(sun_cosine, -1.0f, 1.0f));
+    float sep = horizon_zenith - sun_zenith;
     float n = clampf(sep / fmaxf(angular_radius, 1.0e-6f), -1.0f, 1.0f);
-    // Outside the penumbra n is exactly +-1 and the visible fraction exactly
-    // 1 or 0 (acos(-1) / pi and acos(1) / pi); only inside it does acos(-n)
-    // need evaluating, where its terms cancel.
     if (n >= 1.0f)
         g.visibility = 1.0f;
     else if (n <= -1.0f)
         g.visibility = 0.0f;
     else
         g.visibility = (cr_acos(-n) + n * sqrtf(fmaxf(1.0f - n * n, 0.0f))) / k.pi;
+    float rep = fminf(sun_zenith, horizon_zenith - angular_radius * 0.25f);
+    g.rc = cr_cos(fmaxf(rep, 0.0f));
     return g;
 }
 
@@ -600,9 +659,9 @@ __device__ __forceinline__ SunGeom sun_geometry(const Consts& k, float h, float 
 __device__ __forceinline__ f3 solar_transmittance(const Consts& k, const float* __restrict__ tlut,
                                                   const SunGeom& g, float h, float bottom,
                                                   float angular_radius) {
-    float rep = fminf(g.sun_zenith, g.horizon_zenith - angular_radius * 0.25f);
-    float rc = cr_cos(fmaxf(rep, 0.0f));
-    return sample_transmittance(k, tlut, h, rc, bottom) * g.visibility;
+    if (g.visibility == 0.0f) return mk(0.0f, 0.0f, 0.0f);
+    f3 t = sample_transmittance(k, tlut, h, g.rc, bottom);
+    return g.fast ? t : t * g.visibility;
 }
 
 // renderer.py:791-793
@@ -649,9 +708,10 @@ __device__ __forceinline__ void segment_integral(f3 ext, float step, f3& trans, 
     integ.y = ext.y > 1.0e-12f ? (float)(-expm1(-(double)x.y)) / ext.y : step;
     integ.z = ext.z > 1.0e-12f ? (float)(-expm1(-(double)x.z)) / ext.z : step;
 #else
-    integ.x = ext.x > 1.0e-12f ? (1.0f - trans.x) / ext.x : step;
-    integ.y = ext.y > 1.0e-12f ? (1.0f - trans.y) / ext.y : step;
-    integ.z = ext.z > 1.0e-12f ? (1.0f - trans.z) / ext.z : step;
+    // (1 - T) * rcp(sigma): within 2 ulp of the quotient, not amplified
+    integ.x = ext.x > 1.0e-12f ? (1.0f - trans.x) * __builtin_amdgcn_rcpf(ext.x) : step;
+    integ.y = ext.y > 1.0e-12f ? (1.0f - trans.y) * __builtin_amdgcn_rcpf(ext.y) : step;
+    integ.z = ext.z > 1.0e-12f ? (1.0f - trans.z) * __builtin_amdgcn_rcpf(ext.z) : step;
 #endif
 }
 
@@ -747,9 +807,10 @@ __device__ __forceinline__ bool surface_inscatter_t(const Consts& k, const float
     const float oc = dot(origin / origin_radius, ray);
     const RayPhase ph = ray_phase(k, ray, sun);
     const int n = k.hz_steps;
-    float d0 = boundary(w, 0.0f / (float)n);
+    const float fn = (float)n, rn = 1.0f / fn;
+    float d0 = boundary(w, 0.0f / fn);
     for (int i = 0; i < n; ++i) {
-        float d1 = boundary(w, (float)(i + 1) / (float)n);
+        float d1 = boundary(w, quot((float)(i + 1), fn, rn));
         // _terminator_refinement_weight
         float mid = (d0 + d1) * 0.5f;
         float width = fmaxf(d1 - d0, 1.0e-4f);

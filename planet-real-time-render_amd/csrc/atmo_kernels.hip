@@ -128,6 +128,11 @@ Consts make_consts(const AtmoParams& p) {
     k.mie_b = (float)(2.0 * g);
     k.mie_c = (float)(3.0 * (1.0 - g * g));
     k.mie_d = (float)(8.0 * pi * (2.0 + g * g));
+    // correctly rounded float reciprocals of the divisors of quot()
+    k.r_h_r = 1.0f / k.h_r;
+    k.r_h_m = 1.0f / k.h_m;
+    k.r_half_width = 1.0f / k.half_width;
+    k.r_top = 1.0f / k.top;
     k.tw = p.transmittance_lut_width;
     k.th = p.transmittance_lut_height;
     k.mw = p.multi_scattering_lut_width;

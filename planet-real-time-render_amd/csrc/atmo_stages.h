@@ -70,6 +70,7 @@ __device__ __forceinline__ void transmittance_body(const Consts& k, float bottom
     f3 ray = mk(sine, c, 0.0f);
     Warp w = make_warp(-dot(origin, ray), 0.0f, distance);
     const int steps = k.t_steps;
+    const float fsteps = (float)steps, rsteps = 1.0f / fsteps;
     const int per = (steps + G - 1) / G;
     const int s0 = min(lane * per, steps), s1 = min(s0 + per, steps);
     f3 od = mk(0.0f, 0.0f, 0.0f);
@@ -77,8 +78,8 @@ __device__ __forceinline__ void transmittance_body(const Consts& k, float bottom
     f3 carry = mk(0.0f, 0.0f, 0.0f);  // Kahan compensation of the lane's chunk
 #endif
     for (int s = s0; s < s1; ++s) {
-        float d0 = boundary(w, (float)s / (float)steps);
-        float d1 = boundary(w, (float)(s + 1) / (float)steps);
+        float d0 = boundary(w, quot((float)s, fsteps, rsteps));
+        float d1 = boundary(w, quot((float)(s + 1), fsteps, rsteps));
         float step = d1 - d0;
         float dist = (d0 + d1) * 0.5f;
 #if ATMO_ROBUST
@@ -185,9 +186,10 @@ __device__ __forceinline__ void multi_scattering_body(const Consts& k, float bot
             if (valid && s0 < s1) {
                 Warp w = make_warp(-dot(origin, ray), start, end);
                 RayPhase ph = ray_phase(k, ray, sun);
-                float d0 = boundary(w, (float)s0 / (float)S);
+                const float fS = (float)S, rS = 1.0f / fS;
+                float d0 = boundary(w, quot((float)s0, fS, rS));
                 for (int s = s0; s < s1; ++s) {
-                    float d1 = boundary(w, (float)(s + 1) / (float)S);
+                    float d1 = boundary(w, quot((float)(s + 1), fS, rS));
                     float dist = (d0 + d1) * 0.5f;
                     float step = d1 - d0;
                     f3 point = origin + ray * dist;
@@ -288,11 +290,12 @@ __device__ __forceinline__ void sky_view_body(const Consts& k, float bottom, flo
         Warp w = make_warp(-dot(origin, ray), start, end);
         RayPhase ph = ray_phase(k, ray, sun);
         const int steps = k.sv_steps;
+        const float fsteps = (float)steps, rsteps = 1.0f / fsteps;
         const int per = (steps + G - 1) / G;
         const int s0 = min(lane * per, steps), s1 = min(s0 + per, steps);
         for (int s = s0; s < s1; ++s) {
-            float d0 = boundary(w, (float)s / (float)steps);
-            float d1 = boundary(w, (float)(s + 1) / (float)steps);
+            float d0 = boundary(w, quot((float)s, fsteps, rsteps));
+            float d1 = boundary(w, quot((float)(s + 1), fsteps, rsteps));
             float dist = (d0 + d1) * 0.5f;
             float step = d1 - d0;
             f3 point = origin + ray * dist;

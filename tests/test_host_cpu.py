@@ -247,3 +247,24 @@ def test_view_basis_is_orthonormal(scene):
     np.testing.assert_allclose(m @ m.T, np.eye(3), atol=1e-12)
     np.testing.assert_allclose(np.cross(r, u), f, atol=1e-12)
     assert f[1] == pytest.approx(math.sin(math.radians(-12.0)))
+
+
+def test_markstein_quotient_equals_ieee_division():
+    """The device's quot(a, b, rb) = fma(fma(-q, b, a), rb, q) with
+    q = a * rb and rb = 1.0f / b reproduces the IEEE float quotient a / b
+    (the kernels use it for divisions by per-frame constants).  Emulated
+    here in float64: the inner fma's exact value a - q b fits float64, so
+    rounding it to float32 is the fma's single rounding; the outer sum is
+    rounded twice, which differs from one rounding with probability ~2^-29
+    per sample.  Random operands, including divisors unlike this model's."""
+    g = np.random.default_rng(7)
+    a = np.concatenate([g.uniform(0, 1.2e5, 100_000), np.exp(g.uniform(-20, 15, 100_000)),
+                        np.arange(0, 129)]).astype(np.float32)
+    for b in np.concatenate([[8000, 1200, 15000, 1e5, 800, 120, 1e4, 12, 40, 30, 20, 4],
+                             np.exp(g.uniform(-3, 14, 60))]).astype(np.float32):
+        rb = np.float32(np.float32(1.0) / b)
+        q = (a * rb).astype(np.float32)
+        rem = (a.astype(np.float64) - q.astype(np.float64) * np.float64(b)).astype(np.float32)
+        q1 = (q.astype(np.float64) + rem.astype(np.float64) * np.float64(rb)).astype(np.float32)
+        want = (a / b).astype(np.float32)
+        assert np.array_equal(q1, want), (float(b), int(np.sum(q1 != want)))
