@@ -92,6 +92,13 @@ __device__ __forceinline__ float quot(float a, float b, float rb) {
     const float q = a * rb;
     return fmaf(fmaf(-q, b, a), rb, q);
 }
+
+// a / b elementwise (Taichi's vector / scalar), correctly rounded, with one
+// IEEE reciprocal and three quot().
+__device__ __forceinline__ f3 div3(f3 a, float b) {
+    const float rb = 1.0f / b;
+    return f3{quot(a.x, b, rb), quot(a.y, b, rb), quot(a.z, b, rb)};
+}
 __device__ __forceinline__ f3 maxs(f3 a, float s) { return f3{fmaxf(a.x, s), fmaxf(a.y, s), fmaxf(a.z, s)}; }
 __device__ __forceinline__ f3 mins(f3 a, float s) { return f3{fminf(a.x, s), fminf(a.y, s), fminf(a.z, s)}; }
 __device__ __forceinline__ f3 clamp3(f3 a, float lo, float hi) {
@@ -382,7 +389,8 @@ __device__ __forceinline__ void transmittance_ray_to_uv(const Consts& k, float h
     float dmin = top - h;
     float dmax = rho + sh;
     u = clampf((d - dmin) / fmaxf(dmax - dmin, 1.0e-6f), 0.0f, 1.0f);
-    v = clampf(rho / fmaxf(sh, 1.0e-6f), 0.0f, 1.0f);
+    const float shm = fmaxf(sh, 1.0e-6f);
+    v = clampf(quot(rho, shm, 1.0f / shm), 0.0f, 1.0f);
 }
 
 // Warped interval boundaries (renderer.py:454-480).  The closest-approach
@@ -738,7 +746,7 @@ __device__ __forceinline__ f3 scattering_source(const Consts& k, const float* __
                                                 float h, f3 d, const RayPhase& ph, f3 sun,
                                                 f3 irradiance, float bottom, float ar) {
     float radius = norm(point);
-    f3 radial = point / fmaxf(radius, 1.0f);
+    f3 radial = div3(point, fmaxf(radius, 1.0f));
     float sun_cosine = dot(radial, sun);
     SunGeom g = sun_geometry(k, h, sun_cosine, bottom, ar);
     f3 st = solar_transmittance(k, tlut, g, h, bottom, ar);
@@ -814,14 +822,18 @@ __device__ __forceinline__ bool surface_inscatter_t(const Consts& k, const float
         // _terminator_refinement_weight
         float mid = (d0 + d1) * 0.5f;
         float width = fmaxf(d1 - d0, 1.0e-4f);
-        float closest = 1.0e30f;
-        if (v0) closest = fminf(closest, fabsf(mid - r0));
-        if (v1) closest = fminf(closest, fabsf(mid - r1));
-        float prox = clampf(2.0f - closest / width, 0.0f, 1.0f);
-        prox = prox * prox * (3.0f - 2.0f * prox);
+        // with neither shadow root the proximity term is exactly 0
+        float prox = 0.0f;
+        if (v0 || v1) {
+            float closest = 1.0e30f;
+            if (v0) closest = fminf(closest, fabsf(mid - r0));
+            if (v1) closest = fminf(closest, fabsf(mid - r1));
+            prox = clampf(2.0f - closest / width, 0.0f, 1.0f);
+            prox = prox * prox * (3.0f - 2.0f * prox);
+        }
         f3 point = origin + ray * mid;
         float radius = fmaxf(norm(point), 1.0f);
-        f3 radial = point / radius;
+        f3 radial = div3(point, radius);
         float alt = altitude_on_ray(h0, oc, mid, bottom);
         float sun_cosine = dot(radial, sun);
         SunGeom g = sun_geometry(k, alt, sun_cosine, bottom, ar);
@@ -895,8 +907,8 @@ __device__ __forceinline__ f3 surface_transmittance(const Consts& k,
         f3 near_point = origin + ray * start;
         f3 surface_point = origin + ray * end;
         f3 reverse = -ray;
-        f3 near_radial = near_point / fmaxf(norm(near_point), 1.0f);
-        f3 surface_radial = surface_point / fmaxf(norm(surface_point), 1.0f);
+        f3 near_radial = div3(near_point, fmaxf(norm(near_point), 1.0f));
+        f3 surface_radial = div3(surface_point, fmaxf(norm(surface_point), 1.0f));
         float near_alt = altitude_on_ray(h0, ray.y, start, bottom);
         float surface_alt = altitude_on_ray(h0, ray.y, end, bottom);
         f3 s2t = sample_transmittance(k, tlut, surface_alt, dot(surface_radial, reverse), bottom);
@@ -920,7 +932,7 @@ __device__ __forceinline__ f3 camera_transmittance(const Consts& k,
         } else {
             f3 origin = mk(0.0f, bottom + h, 0.0f);
             f3 entry = origin + ray * start;
-            f3 radial = entry / fmaxf(norm(entry), 1.0f);
+            f3 radial = div3(entry, fmaxf(norm(entry), 1.0f));
             float ea = altitude_on_ray(h, ray.y, start, bottom);
             t = sample_transmittance(k, tlut, ea, dot(radial, ray), bottom);
         }

@@ -480,7 +480,7 @@ __global__ void __launch_bounds__(256) k_prepare(Consts k, long npix,
         f3 vp = ld3(position_view, i);
         f3 local = (right * vp.x + up * vp.y) + forward * vp.z;
         f3 planet = camera + local;
-        f3 radial = planet / fmaxf(norm(planet), 1.0f);
+        f3 radial = div3(planet, fmaxf(norm(planet), 1.0f));
         float sd = norm(local);
         f3 local_ray = local / fmaxf(sd, 1.0f);
         float sc = dot(radial, sun);
