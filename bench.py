@@ -221,6 +221,7 @@ def bench_batch(pkg, scene, device, steps: int):
     states = scene.batch_states(256)
     b = batch.LutBatch(states, 1920, 1080, device)
     stream = torch.cuda.current_stream(device)
+    tuning = b.autotune(stream)
     for _ in range(3):
         b.run(stream)
     torch.cuda.synchronize(device)
@@ -234,6 +235,7 @@ def bench_batch(pkg, scene, device, steps: int):
     b.close()
     return {"states": len(states), "ms_per_batch": round(ms, 4),
             "states_per_s": round(len(states) / ms * 1e3, 1),
+            "lanes_per_texel_t_sv": list(tuning),
             "luts": "T 256x64, MS 32x32, SV 192x108, AP 32x32x32 per state; half Earth, "
                     "half 50 km planet"}
 
@@ -261,6 +263,11 @@ def main() -> int:
         f"sun y {inputs.sun_local[1]:.3f}")
     frame = PlainFrame(pkg, inputs, device)
     stream = torch.cuda.current_stream(device)
+    # one-time set-up: pick the T and sky-view launch shapes for this frame
+    tuning = frame.r.autotune(inputs.planet_radius_m, inputs.camera_radius_m, inputs.sun_local,
+                              inputs.lighting.solar_irradiance,
+                              inputs.lighting.sun_angular_radius_degrees)
+    log(f"autotuned lanes per texel: transmittance {tuning[0]}, sky-view {tuning[1]}")
 
     plain_stages = frame.stages()
     ms_per_step, frame_ms, stage_ms = time_frames(plain_stages, args.steps, args.warmup, stream,
@@ -409,6 +416,7 @@ def main() -> int:
         "plain_frame_ms_median": round(float(np.median(frame_ms)), 4),
         "plain_frame_ms_p99": round(float(np.percentile(frame_ms, 99)), 4),
         "plain_stage_us": {k: round(v * 1e3, 2) for k, v in stage_ms.items()},
+        "lanes_per_texel_t_sv": list(tuning),
         "surface_pixels": n_surface,
         "sky_pixels": n_sky,
         "camera_altitude_m": round(inputs.camera_altitude_m, 3),
