@@ -3,10 +3,12 @@
 // Each __device__ function computes what the @ti.func of the same name in
 // the reference's src/planet_renderer/atmosphere/renderer.py computes (line
 // ranges cited per function), in float32 with the reference's operation
-// order.  The library is compiled with -ffp-contract=off and IEEE division
-// and square root, so the only source of difference from an IEEE float32
-// evaluation of the reference formulas is the ulp-level error of the
-// device exp/acos/cos/sin/pow/atan2.
+// order.  The library is compiled with -ffp-contract=off, and every
+// division and square root is correctly rounded (the compiler's IEEE
+// expansions, or the shorter csqrt/crcp/cdiv/quot forms below, which give
+// the same bits on their domains), so the only source of difference from an
+// IEEE float32 evaluation of the reference formulas is the ulp-level error
+// of the device exp/acos/cos/sin/pow/atan2.
 //
 // Elementary functions are correctly rounded float32 (cr_*: evaluated in
 // float64, rounded once), the definition the CPU restatement uses, wherever
@@ -62,7 +64,13 @@ __device__ __forceinline__ float cr_exp(float x) { return (float)exp((double)x);
 __device__ __forceinline__ float cr_acos(float x) { return (float)acos((double)x); }
 __device__ __forceinline__ float cr_cos(float x) { return (float)cos((double)x); }
 __device__ __forceinline__ float cr_sin(float x) { return (float)sin((double)x); }
-__device__ __forceinline__ float cr_pow(float x, float y) { return (float)pow((double)x, (double)y); }
+// x^1.5 as x sqrt(x) in float64, rounded once: like a float64 pow rounded
+// to float32, it is the correctly rounded value unless x^1.5 lies within a
+// few float64 ulps of a float32 rounding boundary.
+__device__ __forceinline__ float cr_pow15(float x) {
+    const double d = x;
+    return (float)(d * sqrt(d));
+}
 __device__ __forceinline__ float cr_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
 
 __device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
@@ -74,30 +82,67 @@ __device__ __forceinline__ f3 operator*(f3 a, float s) { return f3{a.x * s, a.y 
 __device__ __forceinline__ f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
 __device__ __forceinline__ f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
 __device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
-__device__ __forceinline__ float norm(f3 a) { return sqrtf(dot(a, a)); }
-// Taichi: normalized(v) = (1 / norm(v)) * v.
-__device__ __forceinline__ f3 normalized(f3 a) {
-    float inv = 1.0f / norm(a);
-    return f3{inv * a.x, inv * a.y, inv * a.z};
-}
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(hi, fmaxf(lo, x)); }
+
+// Short forms of the correctly rounded square root and quotient.  The
+// compiler's IEEE float sqrt and division expand to 16 and 11 instructions,
+// most of them for arguments this model never produces (subnormals,
+// infinities, overflowing quotients); the forms below return the same bits
+// on their stated domains in 6 instructions each, without branches.
+
+// sqrt(x): y = v_rsq_f32(max(x, 2^-96)), g = x y, and one residual
+// correction g + (x - g^2) y / 2.  Equal to the IEEE sqrtf for x = +-0,
+// NaN and every float in [2^-96, 2^126) (all 2^32 inputs compared on
+// gfx950).  On (0, 2^-96) it may differ by an ulp of a root below 2^-48;
+// negative x is outside its domain (every call site clamps at 0).  The
+// arguments here are squared planet-scale lengths and differences of
+// cosines, 0 or far above 2^-96.
+__device__ __forceinline__ float csqrt(float x) {
+    const float y = __builtin_amdgcn_rsqf(fmaxf(x, 0x1p-96f));
+    const float g = x * y;
+    return fmaf(fmaf(-g, g, x), 0.5f * y, g);
+}
+
+// 1 / b correctly rounded: v_rcp_f32 and one Newton step y + y (1 - b y).
+// Equal to the IEEE 1.0f / b for every |b| in [2^-100, 2^101) (every
+// mantissa of those 201 binades, both signs, compared on gfx950).  The
+// caller guarantees the domain.
+__device__ __forceinline__ float crcp(float b) {
+    const float y = __builtin_amdgcn_rcpf(b);
+    return fmaf(fmaf(-b, y, 1.0f), y, y);
+}
 
 // a / b, the correctly rounded quotient, for a divisor with a precomputed
 // rb = 1.0f / b (itself correctly rounded): q = a rb is within an ulp, the
 // remainder a - q b is exact in one FMA, and one Markstein correction
-// q + (a - q b) rb rounds to the quotient.  Three instructions instead of
-// the ten of an IEEE division; equal to a / b for every float a in
-// [1e-9, 4e6] and the divisors of this model (checked exhaustively).
+// q + (a - q b) rb rounds to the quotient (Markstein's theorem, with no
+// underflow).  Three instructions instead of the eleven of an IEEE
+// division; equal to a / b for every float a in [1e-9, 4e6] and the
+// constant divisors of this model (checked exhaustively).
 __device__ __forceinline__ float quot(float a, float b, float rb) {
     const float q = a * rb;
     return fmaf(fmaf(-q, b, a), rb, q);
 }
 
-// a / b elementwise (Taichi's vector / scalar), correctly rounded, with one
-// IEEE reciprocal and three quot().
+// a / b correctly rounded, for |b| in [2^-60, 2^60] and |a| <= 2^60 (the
+// caller guarantees both: planet-scale lengths and cosines): quot() with
+// the reciprocal from crcp().  Equal to the IEEE a / b (2^33 random pairs
+// with both exponents in [-60, 60] compared on gfx950, and Markstein's
+// theorem) except that -0 / b gives +0 and a subnormal quotient may differ
+// by an ulp.
+__device__ __forceinline__ float cdiv(float a, float b) { return quot(a, b, crcp(b)); }
+
+// a / b elementwise (Taichi's vector / scalar), correctly rounded, for
+// b in [1, 2^60] (radii clamped below at 1 m): one crcp() and three quot().
 __device__ __forceinline__ f3 div3(f3 a, float b) {
-    const float rb = 1.0f / b;
+    const float rb = crcp(b);
     return f3{quot(a.x, b, rb), quot(a.y, b, rb), quot(a.z, b, rb)};
+}
+__device__ __forceinline__ float norm(f3 a) { return csqrt(dot(a, a)); }
+// Taichi: normalized(v) = (1 / norm(v)) * v.
+__device__ __forceinline__ f3 normalized(f3 a) {
+    float inv = 1.0f / norm(a);
+    return f3{inv * a.x, inv * a.y, inv * a.z};
 }
 __device__ __forceinline__ f3 maxs(f3 a, float s) { return f3{fmaxf(a.x, s), fmaxf(a.y, s), fmaxf(a.z, s)}; }
 __device__ __forceinline__ f3 mins(f3 a, float s) { return f3{fminf(a.x, s), fminf(a.y, s), fminf(a.z, s)}; }
@@ -191,7 +236,7 @@ __device__ __forceinline__ void sphere_roots(const Consts& k, f3 origin, f3 ray,
     far = -k.large;
     if (hit) {
         float disc = fmaxf(clearance, 0.0f) * (radius + perpendicular);
-        float root = sqrtf(fmaxf(disc, 0.0f));
+        float root = csqrt(fmaxf(disc, 0.0f));
         near = -projected - root;
         far = -projected + root;
     }
@@ -200,7 +245,7 @@ __device__ __forceinline__ void sphere_roots(const Consts& k, f3 origin, f3 ray,
 // renderer.py:228-235
 __device__ __forceinline__ float shell_rho(float altitude, float bottom) {
     float s = fmaxf(altitude, 0.0f);
-    return sqrtf(s * (2.0f * bottom + s));
+    return csqrt(s * (2.0f * bottom + s));
 }
 
 // renderer.py:237-262
@@ -217,8 +262,8 @@ __device__ __forceinline__ float altitude_on_ray(float h0, float c, float t, flo
     float a = fmaxf(h0, 0.0f);
     float r0 = bottom + a;
     float delta = a * (2.0f * bottom + a) + 2.0f * r0 * c * t + t * t;
-    float len = sqrtf(fmaxf(bottom * bottom + delta, 0.0f));
-    return delta / fmaxf(len + bottom, 1.0f);
+    float len = csqrt(fmaxf(bottom * bottom + delta, 0.0f));
+    return cdiv(delta, fmaxf(len + bottom, 1.0f));
 }
 
 // renderer.py:264-301
@@ -256,14 +301,14 @@ __device__ __forceinline__ void shell_roots(const Consts& k, float h0, float c, 
     near = k.large;
     far = -k.large;
     if (hit) {
-        float root = sqrtf(fmaxf(disc, 0.0f));
+        float root = csqrt(fmaxf(disc, 0.0f));
         float sign = hl < 0.0f ? -1.0f : 1.0f;
         float q = -hl - sign * root;
         float root0 = -hl - root;
         float root1 = -hl + root;
         if (fabsf(q) > 1.0e-12f) {
             root0 = q;
-            root1 = constant / q;
+            root1 = cdiv(constant, q);
         }
         near = fminf(root0, root1);
         far = fmaxf(root0, root1);
@@ -348,7 +393,7 @@ __device__ __forceinline__ float distance_to_top(const Consts& k, float h, float
     float radius = bottom + fmaxf(h, 0.0f);
     float shell_delta = (k.top - h) * (2.0f * bottom + k.top + h);
     float disc = radius * radius * c * c + shell_delta;
-    return fmaxf(-radius * c + sqrtf(fmaxf(disc, 0.0f)), 0.0f);
+    return fmaxf(-radius * c + csqrt(fmaxf(disc, 0.0f)), 0.0f);
 }
 
 // renderer.py:410-432
@@ -358,8 +403,8 @@ __device__ __forceinline__ void transmittance_uv_to_ray(const Consts& k, float u
     float top = k.top;
     float sh = shell_rho(top, bottom);
     float rho = sh * clampf(v, 0.0f, 1.0f);
-    float rl = sqrtf(rho * rho + bottom * bottom);
-    float alt = rho * rho / fmaxf(rl + bottom, 1.0f);
+    float rl = csqrt(rho * rho + bottom * bottom);
+    float alt = cdiv(rho * rho, fmaxf(rl + bottom, 1.0f));
     float radius = bottom + alt;
     float dmin = top - alt;
     float dmax = rho + sh;
@@ -371,7 +416,7 @@ __device__ __forceinline__ void transmittance_uv_to_ray(const Consts& k, float u
         cs = (float)((((double)top - alt) * (2.0 * (double)bottom + top + alt) - (double)d * d) /
                      (2.0 * (double)radius * d));
 #else
-    if (d > 1.0e-6f) cs = ((top - alt) * (2.0f * bottom + top + alt) - d * d) / (2.0f * radius * d);
+    if (d > 1.0e-6f) cs = cdiv((top - alt) * (2.0f * bottom + top + alt) - d * d, 2.0f * radius * d);
 #endif
     altitude = alt;
     cosine = clampf(cs, -1.0f, 1.0f);
@@ -388,9 +433,9 @@ __device__ __forceinline__ void transmittance_ray_to_uv(const Consts& k, float h
     float d = distance_to_top(k, h, c, bottom);
     float dmin = top - h;
     float dmax = rho + sh;
-    u = clampf((d - dmin) / fmaxf(dmax - dmin, 1.0e-6f), 0.0f, 1.0f);
+    u = clampf(cdiv(d - dmin, fmaxf(dmax - dmin, 1.0e-6f)), 0.0f, 1.0f);
     const float shm = fmaxf(sh, 1.0e-6f);
-    v = clampf(quot(rho, shm, 1.0f / shm), 0.0f, 1.0f);
+    v = clampf(quot(rho, shm, crcp(shm)), 0.0f, 1.0f);
 }
 
 // Warped interval boundaries (renderer.py:454-480).  The closest-approach
@@ -438,7 +483,7 @@ __device__ __forceinline__ float boundary(const Warp& w, float f) {
 __device__ __forceinline__ float sky_horizon_zenith(const Consts& k, float h, float bottom) {
     float a = fmaxf(h, 0.0f);
     float radius = bottom + a;
-    float tc = shell_rho(a, bottom) / fmaxf(radius, 1.0f);
+    float tc = cdiv(shell_rho(a, bottom), fmaxf(radius, 1.0f));
     return k.pi - cr_acos(clampf(tc, 0.0f, 1.0f));
 }
 
@@ -455,11 +500,11 @@ __device__ __forceinline__ float sky_v_to_zenith(const Consts& k, float v, float
 // renderer.py:506-520
 __device__ __forceinline__ float sky_zenith_to_v(const Consts& k, float zenith, float horizon) {
     if (zenith < horizon) {
-        float n = clampf(zenith / fmaxf(horizon, 1.0e-6f), 0.0f, 1.0f);
-        return 0.5f * (1.0f - sqrtf(fmaxf(1.0f - n, 0.0f)));
+        float n = clampf(cdiv(zenith, fmaxf(horizon, 1.0e-6f)), 0.0f, 1.0f);
+        return 0.5f * (1.0f - csqrt(fmaxf(1.0f - n, 0.0f)));
     }
-    float n = clampf((zenith - horizon) / fmaxf(k.pi - horizon, 1.0e-6f), 0.0f, 1.0f);
-    return 0.5f + 0.5f * sqrtf(n);
+    float n = clampf(cdiv(zenith - horizon, fmaxf(k.pi - horizon, 1.0e-6f)), 0.0f, 1.0f);
+    return 0.5f + 0.5f * csqrt(n);
 }
 
 #if ATMO_ROBUST
@@ -499,7 +544,7 @@ __device__ __forceinline__ double sky_v_to_zenith_w(float v, float h, float bott
 // renderer.py:522-542
 __device__ __forceinline__ float sun_u_to_cosine(float u, float h, float bottom) {
     float radius = bottom + fmaxf(h, 0.0f);
-    float hc = -shell_rho(h, bottom) / fmaxf(radius, 1.0f);
+    float hc = cdiv(-shell_rho(h, bottom), fmaxf(radius, 1.0f));
     float c;
     if (u < 0.5f) {
         float inv = 1.0f - 2.0f * u;
@@ -514,13 +559,13 @@ __device__ __forceinline__ float sun_u_to_cosine(float u, float h, float bottom)
 // renderer.py:544-568
 __device__ __forceinline__ float sun_cosine_to_u(float c, float h, float bottom) {
     float radius = bottom + fmaxf(h, 0.0f);
-    float hc = -shell_rho(h, bottom) / fmaxf(radius, 1.0f);
+    float hc = cdiv(-shell_rho(h, bottom), fmaxf(radius, 1.0f));
     if (c < hc) {
-        float n = clampf((hc - c) / fmaxf(hc + 1.0f, 1.0e-6f), 0.0f, 1.0f);
-        return 0.5f * (1.0f - sqrtf(n));
+        float n = clampf(cdiv(hc - c, fmaxf(hc + 1.0f, 1.0e-6f)), 0.0f, 1.0f);
+        return 0.5f * (1.0f - csqrt(n));
     }
-    float n = clampf((c - hc) / fmaxf(1.0f - hc, 1.0e-6f), 0.0f, 1.0f);
-    return 0.5f + 0.5f * sqrtf(n);
+    float n = clampf(cdiv(c - hc, fmaxf(1.0f - hc, 1.0e-6f)), 0.0f, 1.0f);
+    return 0.5f + 0.5f * csqrt(n);
 }
 
 // ------------------------------------------------------------- sampling
@@ -555,7 +600,7 @@ __device__ __forceinline__ f3 sample_multi_scattering(const Consts& k,
                                                       float sun_cosine, float bottom) {
     float af = clampf(quot(h, k.top, k.r_top), 0.0f, 1.0f);
     float u = sun_cosine_to_u(sun_cosine, fmaxf(h, 0.0f), bottom);
-    float v = sqrtf(af);
+    float v = csqrt(af);
     return bilinear(mslut, k.mw, k.mh, u, v);
 }
 
@@ -576,9 +621,9 @@ __device__ __forceinline__ void solar_shadow_roots(f3 origin, f3 ray, f3 sun, fl
     v1 = false;
     float disc = hl * hl - quadratic * constant;
     if (quadratic > 1.0e-8f && disc >= 0.0f) {
-        float sr = sqrtf(fmaxf(disc, 0.0f));
-        r0 = (-hl - sr) / quadratic;
-        r1 = (-hl + sr) / quadratic;
+        float sr = csqrt(fmaxf(disc, 0.0f));
+        r0 = cdiv(-hl - sr, quadratic);
+        r1 = cdiv(-hl + sr, quadratic);
         f3 p0 = origin + ray * r0;
         f3 p1 = origin + ray * r1;
         v0 = dot(p0, sun) < 0.0f;
@@ -607,12 +652,18 @@ __device__ __forceinline__ void solar_shadow_roots(f3 origin, f3 ray, f3 sun, fl
 // float(cos(float(acos(x)))), both roundings correct, with one float64
 // acos: cos(acos(x) + e) = x cos e - sqrt(1 - x^2) sin e with
 // |e| < 2^-22, so cos e = 1 - O(1e-14) and sin e = e to far below float32
-// rounding.
+// rounding.  sqrt(1 - x^2) multiplies e, so it needs far less than
+// float64 precision: a float rsqrt seed and one float64 Newton correction
+// give it to ~2^-45, which moves x - sqrt(1 - x^2) e by < 1e-21.
 __device__ __forceinline__ float cos_of_rounded_acos(float x) {
     const double xd = fmin(fmax((double)x, -1.0), 1.0);
     const double t = acos(xd);
     const double e = (double)(float)t - t;
-    return (float)(xd - sqrt(fmax(1.0 - xd * xd, 0.0)) * e);
+    const double v = fmax(1.0 - xd * xd, 0.0);
+    const double y = (double)__builtin_amdgcn_rsqf(fmaxf((float)v, 0x1p-96f));
+    const double g = v * y;
+    const double sine = fma(fma(-g, g, v), 0.5 * y, g);
+    return (float)(xd - sine * e);
 }
 
 struct SunGeom {
@@ -625,11 +676,11 @@ __device__ __forceinline__ SunGeom sun_geometry(const Consts& k, float h, float 
                                                 float bottom, float angular_radius) {
     SunGeom g;
     float radius = bottom + fmaxf(h, 0.0f);
-    float hc = -shell_rho(h, bottom) / fmaxf(radius, 1.0f);
+    float hc = cdiv(-shell_rho(h, bottom), fmaxf(radius, 1.0f));
     {
         const float dl = 1.05f * angular_radius;
         const float cd = 1.0f - 0.5f * dl * dl, sd = dl;  // cos d, sin d to O(d^3)
-        const float hs = sqrtf(fmaxf(1.0f - hc * hc, 0.0f));
+        const float hs = csqrt(fmaxf(1.0f - hc * hc, 0.0f));
         const float up = hc * cd + hs * sd, down = hc * cd - hs * sd;
         if (sun_cosine <= down) {
             g.fast = true;
@@ -651,13 +702,13 @@ __device__ __forceinline__ SunGeom sun_geometry(const Consts& k, float h, float 
     const float horizon_zenith = cr_acos(clampf(hc, -1.0f, 1.0f));
     const float sun_zenith = cr_acos(clampf(sun_cosine, -1.0f, 1.0f));
     float sep = horizon_zenith - sun_zenith;
-    float n = clampf(sep / fmaxf(angular_radius, 1.0e-6f), -1.0f, 1.0f);
+    float n = clampf(cdiv(sep, fmaxf(angular_radius, 1.0e-6f)), -1.0f, 1.0f);
     if (n >= 1.0f)
         g.visibility = 1.0f;
     else if (n <= -1.0f)
         g.visibility = 0.0f;
     else
-        g.visibility = (cr_acos(-n) + n * sqrtf(fmaxf(1.0f - n * n, 0.0f))) / k.pi;
+        g.visibility = cdiv(cr_acos(-n) + n * csqrt(fmaxf(1.0f - n * n, 0.0f)), k.pi);
     float rep = fminf(sun_zenith, horizon_zenith - angular_radius * 0.25f);
     g.rc = cr_cos(fmaxf(rep, 0.0f));
     return g;
@@ -674,13 +725,13 @@ __device__ __forceinline__ f3 solar_transmittance(const Consts& k, const float* 
 
 // renderer.py:791-793
 __device__ __forceinline__ float rayleigh_phase(const Consts& k, float c) {
-    return 3.0f * (1.0f + c * c) / k.pi16;
+    return cdiv(3.0f * (1.0f + c * c), k.pi16);
 }
 
 // renderer.py:795-804
 __device__ __forceinline__ float mie_phase(const Consts& k, float c) {
     float den = fmaxf(k.mie_a - k.mie_b * c, 1.0e-8f);
-    return k.mie_c * (1.0f + c * c) / (k.mie_d * cr_pow(den, 1.5f));
+    return cdiv(k.mie_c * (1.0f + c * c), k.mie_d * cr_pow15(den));
 }
 
 // exp(-x) for the segment transmittance, x in [0, 80].  For x < 1/16 the
@@ -812,7 +863,7 @@ __device__ __forceinline__ bool surface_inscatter_t(const Consts& k, const float
     solar_shadow_roots(origin, ray, sun, bottom, r0, r1, v0, v1);
     const Warp w = make_warp(-dot(origin, ray), start, end);
     const float origin_radius = fmaxf(norm(origin), 1.0f);
-    const float oc = dot(origin / origin_radius, ray);
+    const float oc = dot(div3(origin, origin_radius), ray);
     const RayPhase ph = ray_phase(k, ray, sun);
     const int n = k.hz_steps;
     const float fn = (float)n, rn = 1.0f / fn;
@@ -828,7 +879,7 @@ __device__ __forceinline__ bool surface_inscatter_t(const Consts& k, const float
             float closest = 1.0e30f;
             if (v0) closest = fminf(closest, fabsf(mid - r0));
             if (v1) closest = fminf(closest, fabsf(mid - r1));
-            prox = clampf(2.0f - closest / width, 0.0f, 1.0f);
+            prox = clampf(2.0f - cdiv(closest, width), 0.0f, 1.0f);
             prox = prox * prox * (3.0f - 2.0f * prox);
         }
         f3 point = origin + ray * mid;
@@ -967,7 +1018,7 @@ __device__ __forceinline__ f3 sample_aerial(const Consts& k, const float* __rest
     bool valid;
     aerial_prefix(k, h, ray.y, bottom, start, end, valid);
     float df = 0.0f;
-    if (valid) df = clampf((distance - start) / fmaxf(end - start, 1.0f), 0.0f, 1.0f);
+    if (valid) df = clampf(cdiv(distance - start, fmaxf(end - start, 1.0f)), 0.0f, 1.0f);
     const int aw = k.aw, ah = k.ah, ad = k.ad;
     float x = clampf(su * (float)aw - 0.5f, 0.0f, (float)(aw - 1));
     float y = clampf(sv * (float)ah - 0.5f, 0.0f, (float)(ah - 1));

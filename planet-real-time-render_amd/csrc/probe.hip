@@ -5,8 +5,13 @@
 // the issue rate rather than the latency is timed.  atmo_probe_copy is a
 // plain device copy of known size: the reference point that scales the HBM
 // traffic counters (FETCH_SIZE, WRITE_SIZE) of the other kernels.
+// atmo_probe_arith_check compares the short correctly rounded square root,
+// reciprocal and quotient of the atmosphere kernels (atmo::csqrt, crcp,
+// cdiv) with the compiler's IEEE expansions over their stated domains.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "atmo_device.h"
 
 namespace {
 
@@ -34,9 +39,83 @@ __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src,
         dst[i] = src[i];
 }
 
+// counts[0]: csqrt != sqrtf over all 2^32 floats within csqrt's domain
+// (+-0, NaN, [2^-96, 2^126)); counts[1]: crcp != 1.0f / b over
+// every float with |b| in [2^-100, 2^101); counts[2]: cdiv != a / b over
+// hashed pairs with both exponents in [-60, 60] (2^32 pairs); counts[3]:
+// csqrt != sqrtf on (0, 2^-96), beyond 2^126 and for negative x, for
+// information.
+__device__ __forceinline__ bool same_bits(float a, float b) {
+    return __float_as_uint(a) == __float_as_uint(b) || (a != a && b != b);
+}
+
+__global__ void __launch_bounds__(256) k_check_sqrt(unsigned long long* counts, uint32_t base) {
+    const uint32_t bits = base + blockIdx.x * blockDim.x + threadIdx.x;
+    const float x = __uint_as_float(bits);
+    if (same_bits(atmo::csqrt(x), sqrtf(x))) return;
+    const bool domain = x == 0.0f || x != x || (x >= 0x1p-96f && x < 0x1p126f);
+    atomicAdd(&counts[domain ? 0 : 3], 1ull);
+}
+
+__global__ void __launch_bounds__(256) k_check_rcp(unsigned long long* counts, int exponent) {
+    const uint32_t bits = ((uint32_t)(exponent + 127) << 23) | (blockIdx.x * blockDim.x + threadIdx.x);
+    const float b = __uint_as_float(bits);
+    if (!same_bits(atmo::crcp(b), 1.0f / b)) atomicAdd(&counts[1], 1ull);
+    if (!same_bits(atmo::crcp(-b), 1.0f / -b)) atomicAdd(&counts[1], 1ull);
+}
+
+__device__ __forceinline__ uint32_t mix(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return (uint32_t)x;
+}
+
+__global__ void __launch_bounds__(256) k_check_div(unsigned long long* counts, uint64_t base) {
+    const uint64_t i = base + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t ha = mix(2 * i), hb = mix(2 * i + 1);
+    const uint32_t ea = (ha >> 23) % 121u, eb = (hb >> 23) % 121u;
+    const float a = __uint_as_float((ha & 0x807fffffu) | ((ea + 67u) << 23));
+    const float b = __uint_as_float((hb & 0x807fffffu) | ((eb + 67u) << 23));
+    if (!same_bits(atmo::cdiv(a, b), a / b)) atomicAdd(&counts[2], 1ull);
+}
+
 }  // namespace
 
 extern "C" {
+
+// Runs the three comparisons above on the null stream and copies the four
+// mismatch counts to counts[0..3].  Returns 0 on success, otherwise a
+// hipError_t value.
+int32_t atmo_probe_arith_check(uint64_t* counts) {
+    unsigned long long* d = nullptr;
+    hipError_t e = hipMalloc(&d, 4 * sizeof(unsigned long long));
+    if (e != hipSuccess) return (int32_t)e;
+    e = hipMemset(d, 0, 4 * sizeof(unsigned long long));
+    // 2^32 sqrt inputs: 16 launches of 2^20 blocks x 256 threads
+    for (uint32_t c = 0; c < 16 && e == hipSuccess; ++c) {
+        hipLaunchKernelGGL(k_check_sqrt, dim3(1u << 20), dim3(256), 0, 0, d, c << 28);
+        e = hipGetLastError();
+    }
+    // 2^23 mantissas per binade: 2^15 blocks x 256 threads
+    for (int x = -100; x <= 100 && e == hipSuccess; ++x) {
+        hipLaunchKernelGGL(k_check_rcp, dim3(1u << 15), dim3(256), 0, 0, d, x);
+        e = hipGetLastError();
+    }
+    for (uint64_t c = 0; c < 16 && e == hipSuccess; ++c) {
+        hipLaunchKernelGGL(k_check_div, dim3(1u << 20), dim3(256), 0, 0, d, c << 28);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    unsigned long long h[4] = {0, 0, 0, 0};
+    if (e == hipSuccess) e = hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return (int32_t)e;
+    for (int i = 0; i < 4; ++i) counts[i] = (uint64_t)h[i];
+    return 0;
+}
 
 // Copies `bytes` (a multiple of 16) from src to dst on `stream` with
 // k_copy: exactly `bytes` read and `bytes` written.  Returns 0 on success,

@@ -1,0 +1,28 @@
+"""The short correctly rounded square root, reciprocal and quotient of the
+atmosphere kernels (atmo::csqrt, crcp, cdiv) return the bits of the
+compiler's IEEE expansions on their stated domains: every float for the
+square root and the reciprocal, 2^32 hashed pairs for the quotient.  This is
+what lets the kernels use them while keeping the reference's float32
+arithmetic bit for bit."""
+
+import ctypes
+
+import pytest
+
+from conftest import REPO
+
+pytestmark = pytest.mark.gpu
+
+
+def test_short_sqrt_rcp_div_equal_ieee():
+    path = REPO / "planet-real-time-render_amd" / "libatmo_probe.so"
+    assert path.exists(), "libatmo_probe.so is missing: build first"
+    lib = ctypes.CDLL(str(path))
+    lib.atmo_probe_arith_check.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
+    lib.atmo_probe_arith_check.restype = ctypes.c_int32
+    counts = (ctypes.c_uint64 * 4)()
+    print("arith check: 2^32 sqrt inputs, 201 x 2^24 reciprocals, 2^32 quotients", flush=True)
+    assert lib.atmo_probe_arith_check(counts) == 0
+    print(f"mismatches: sqrt {counts[0]}, rcp {counts[1]}, div {counts[2]}; "
+          f"sqrt outside its domain {counts[3]}", flush=True)
+    assert (counts[0], counts[1], counts[2]) == (0, 0, 0)
