@@ -220,6 +220,11 @@ def library() -> ctypes.CDLL:
                 f"{LIBRARY_PATH} is missing: build the HIP library first "
                 "(__graft_entry__.build() or python -m planet_real_time_render_amd.build)"
             )
+        # torch's HIP runtime initialises first: when libatmo.so's HIP
+        # runtime is loaded and used before it, torch finds no device.
+        import torch
+
+        torch.cuda.is_available()
         lib = ctypes.CDLL(str(LIBRARY_PATH))
         _signatures(lib)
         _LIB = lib

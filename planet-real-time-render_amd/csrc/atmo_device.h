@@ -60,8 +60,50 @@ struct f3 {
     float x, y, z;
 };
 
+// acos in float64 for |x| <= 1, within 1.2 ulp: asin(s) = s + s w P(w),
+// w = s^2 <= 1/4, with P the degree-12 Chebyshev fit of
+// (asin(sqrt(w)) - sqrt(w)) / w^1.5 on [0, 1/4] (fit error 1.5e-17);
+// acos(x) = pi/2 - asin(x) for |x| < 1/2, else 2 asin(sqrt((1 - |x|) / 2))
+// reflected for x < 0.  The square root is a float rsqrt seed with two
+// float64 Newton corrections.  Half the instructions of the library acos,
+// whose |x| >= 1/2 branch carries a double-double reconstruction this
+// model does not need: its float64 results are only ever rounded to float32.
+__device__ __forceinline__ double asin_poly(double w) {
+    double p = 0x1.d72b2bc8155f8p-6;
+    p = fma(p, w, -0x1.e6aaa8a0a04ccp-7);
+    p = fma(p, w, 0x1.1d189408314eep-6);
+    p = fma(p, w, 0x1.65a9c4dfcf8b2p-8);
+    p = fma(p, w, 0x1.52420b04b37bep-7);
+    p = fma(p, w, 0x1.782651caa6547p-7);
+    p = fma(p, w, 0x1.c9cf07674736ap-7);
+    p = fma(p, w, 0x1.1c4d35cf95421p-6);
+    p = fma(p, w, 0x1.6e8bb1c8209a2p-6);
+    p = fma(p, w, 0x1.f1c71c1db0623p-6);
+    p = fma(p, w, 0x1.6db6db6e31f13p-5);
+    p = fma(p, w, 0x1.3333333332ecap-4);
+    return fma(p, w, 0x1.5555555555556p-3);
+}
+
+__device__ __forceinline__ double acos_d(double x) {
+    const double pio2_hi = 0x1.921fb54442d18p+0, pio2_lo = 0x1.1a62633145c07p-54;
+    const double pi_hi = 0x1.921fb54442d18p+1, pi_lo = 0x1.1a62633145c07p-53;
+    const double a = fabs(x);
+    if (a < 0.5) {
+        const double w = x * x;
+        return (pio2_hi - x) + (pio2_lo - (x * w) * asin_poly(w));
+    }
+    const double w = fma(a, -0.5, 0.5);  // (1 - |x|) / 2, exact
+    const double y = (double)__builtin_amdgcn_rsqf(fmaxf((float)w, 0x1p-96f));
+    const double h = 0.5 * y;
+    double s = w * y;
+    s = fma(fma(-s, s, w), h, s);
+    s = fma(fma(-s, s, w), h, s);
+    const double q = fma(s * w, asin_poly(w), s);
+    return x > 0.0 ? 2.0 * q : (pi_hi - 2.0 * q) + pi_lo;
+}
+
 __device__ __forceinline__ float cr_exp(float x) { return (float)exp((double)x); }
-__device__ __forceinline__ float cr_acos(float x) { return (float)acos((double)x); }
+__device__ __forceinline__ float cr_acos(float x) { return (float)acos_d((double)x); }
 __device__ __forceinline__ float cr_cos(float x) { return (float)cos((double)x); }
 __device__ __forceinline__ float cr_sin(float x) { return (float)sin((double)x); }
 // x^1.5 as x sqrt(x) in float64, rounded once: like a float64 pow rounded
@@ -657,7 +699,7 @@ __device__ __forceinline__ void solar_shadow_roots(f3 origin, f3 ray, f3 sun, fl
 // give it to ~2^-45, which moves x - sqrt(1 - x^2) e by < 1e-21.
 __device__ __forceinline__ float cos_of_rounded_acos(float x) {
     const double xd = fmin(fmax((double)x, -1.0), 1.0);
-    const double t = acos(xd);
+    const double t = acos_d(xd);
     const double e = (double)(float)t - t;
     const double v = fmax(1.0 - xd * xd, 0.0);
     const double y = (double)__builtin_amdgcn_rsqf(fmaxf((float)v, 0x1p-96f));

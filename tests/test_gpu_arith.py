@@ -15,6 +15,11 @@ pytestmark = pytest.mark.gpu
 
 
 def test_short_sqrt_rcp_div_equal_ieee():
+    # torch first, as everywhere else: its HIP runtime must be the one loaded
+    # before the probe library's dependencies resolve
+    import torch
+
+    assert torch.cuda.is_available()
     path = REPO / "planet-real-time-render_amd" / "libatmo_probe.so"
     assert path.exists(), "libatmo_probe.so is missing: build first"
     lib = ctypes.CDLL(str(path))
