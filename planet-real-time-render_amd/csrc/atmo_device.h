@@ -180,10 +180,16 @@ __device__ __forceinline__ f3 div3(f3 a, float b) {
     const float rb = crcp(b);
     return f3{quot(a.x, b, rb), quot(a.y, b, rb), quot(a.z, b, rb)};
 }
+// 1 / x: crcp() on its domain, the IEEE quotient elsewhere (zero, tiny,
+// huge and non-finite x, on a branch).
+__device__ __forceinline__ float rcp_any(float x) {
+    if (__builtin_expect(fabsf(x) >= 0x1p-100f && fabsf(x) < 0x1p100f, 1)) return crcp(x);
+    return 1.0f / x;
+}
 __device__ __forceinline__ float norm(f3 a) { return csqrt(dot(a, a)); }
 // Taichi: normalized(v) = (1 / norm(v)) * v.
 __device__ __forceinline__ f3 normalized(f3 a) {
-    float inv = 1.0f / norm(a);
+    float inv = rcp_any(norm(a));
     return f3{inv * a.x, inv * a.y, inv * a.z};
 }
 __device__ __forceinline__ f3 maxs(f3 a, float s) { return f3{fmaxf(a.x, s), fmaxf(a.y, s), fmaxf(a.z, s)}; }
@@ -1034,8 +1040,11 @@ __device__ __forceinline__ f3 camera_transmittance(const Consts& k,
 }
 
 // renderer.py:1474-1525
+// `horizon` is sky_horizon_zenith(k, h, bottom), constant per frame and
+// computed once by the caller.
 __device__ __forceinline__ f3 sample_sky_view(const Consts& k, const float* __restrict__ svlut,
-                                              f3 ray, f3 sun, float h, float bottom) {
+                                              f3 ray, f3 sun, float h, float bottom,
+                                              float horizon) {
     float zenith = cr_acos(clampf(ray.y, -1.0f, 1.0f));
     float vl = sqrtf(ray.x * ray.x + ray.z * ray.z);
     float sl = sqrtf(sun.x * sun.x + sun.z * sun.z);
@@ -1045,9 +1054,11 @@ __device__ __forceinline__ f3 sample_sky_view(const Consts& k, const float* __re
     float u = rel / k.pi;
 #if ATMO_ROBUST
     (void)zenith;
+    (void)horizon;
     float v = sky_zenith_to_v_w(acos(fmin(fmax((double)ray.y, -1.0), 1.0)), h, bottom);
 #else
-    float v = sky_zenith_to_v(k, zenith, sky_horizon_zenith(k, h, bottom));
+    (void)h;
+    float v = sky_zenith_to_v(k, zenith, horizon);
 #endif
     return bilinear(svlut, k.sw, k.sh, u, v);
 }
@@ -1081,13 +1092,16 @@ __device__ __forceinline__ f3 sample_aerial(const Consts& k, const float* __rest
 // Sky branch of the COMPOSITE view (renderer.py:1882-1940): sky-view
 // radiance, scene and stars through the camera transmittance with the
 // star-contrast fade, plus the finite sun disk with a one-pixel filter.
+// The frame constants `horizon` (sky_horizon_zenith at the sky-view
+// altitude) and `pixel_filter` (atan2(thf sqrt(2) / height, 1)) come from
+// the caller.
 __device__ __forceinline__ f3 composite_sky(const Consts& k, const float* __restrict__ tlut,
                                             const float* __restrict__ svlut, f3 ray, f3 color,
                                             f3 space, float cam_alt, float sky_alt, f3 sun, f3 disk,
-                                            float ar, float thf, int height, float c0, float c1,
-                                            float bottom) {
+                                            float ar, float pixel_filter, float horizon, float c0,
+                                            float c1, float bottom) {
     f3 trans = camera_transmittance(k, tlut, cam_alt, ray, bottom);
-    f3 sky = sample_sky_view(k, svlut, ray, sun, sky_alt, bottom);
+    f3 sky = sample_sky_view(k, svlut, ray, sun, sky_alt, bottom, horizon);
     f3 star = space * trans;
     const f3 lum = mk(0.2126f, 0.7152f, 0.0722f);
     float contrast = dot(star, lum) / fmaxf(dot(sky, lum), 1.0e-5f);
@@ -1095,22 +1109,27 @@ __device__ __forceinline__ f3 composite_sky(const Consts& k, const float* __rest
     vis = vis * vis * (3.0f - 2.0f * vis);
     color = (sky + color * trans) + star * vis;
     float angular_distance = cr_acos(clampf(dot(ray, sun), -1.0f, 1.0f));
-    float pixel_filter = cr_atan2(thf * 1.41421356f / (float)height, 1.0f);
     float cov = clampf((ar + pixel_filter - angular_distance) / fmaxf(2.0f * pixel_filter, 1.0e-8f),
                        0.0f, 1.0f);
     cov = cov * cov * (3.0f - 2.0f * cov);
     return color + (disk * trans) * cov;
 }
 
-// Pixel-centre ray (renderer.py:1850-1859), y = 0 the bottom row.
-__device__ __forceinline__ f3 camera_ray(int px, int py, int width, int height, f3 right, f3 up,
-                                         f3 forward, float thf) {
-    float aspect = (float)width / (float)height;
-    float su = ((float)px + 0.5f) / (float)width;
-    float sv = ((float)py + 0.5f) / (float)height;
+// Pixel-centre ray (renderer.py:1850-1859), y = 0 the bottom row.  The
+// frame constants aspect = width / height and fwd_t = forward / thf come
+// from the caller, rounded as the reference rounds them.
+__device__ __forceinline__ f3 camera_ray(int px, int py, int width, int height, float aspect,
+                                         f3 right, f3 up, f3 fwd_t, float& su, float& sv) {
+    su = cdiv((float)px + 0.5f, (float)width);
+    sv = cdiv((float)py + 0.5f, (float)height);
     float sx = (su * 2.0f - 1.0f) * aspect;
     float sy = sv * 2.0f - 1.0f;
-    return normalized((right * sx + up * sy) + forward / thf);
+    return normalized((right * sx + up * sy) + fwd_t);
+}
+__device__ __forceinline__ f3 camera_ray(int px, int py, int width, int height, float aspect,
+                                         f3 right, f3 up, f3 fwd_t) {
+    float su, sv;
+    return camera_ray(px, py, width, height, aspect, right, up, fwd_t, su, sv);
 }
 
 // Compacts the flagged threads of a 256-thread block into list[0..count) in

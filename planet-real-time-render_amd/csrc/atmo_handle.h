@@ -56,7 +56,17 @@ struct CompositeArgs {
     int view;
     int* queue;  // COMPOSITE view: deferred pixels, layout of AtmoHandle::queue
     int parity;  // the counter of this call: queue[parity]
+    // frame constants (composite_frame_constants)
+    float aspect;        // width / height
+    atmo::f3 fwd_t;      // forward / thf
+    float pixel_filter;  // atan2(thf * 1.41421356 / height, 1), the sun disk's pixel filter
+    float sky_horizon;   // sky_horizon_zenith at sky_alt
 };
+
+// Fills the frame constants of `a` from its other fields, with the float32
+// operations of the reference's per-pixel expressions (renderer.py:1850-1859,
+// 482-490, 1930-1936) evaluated once on the host (atmo_kernels.hip).
+void composite_frame_constants(const atmo::Consts& k, CompositeArgs& a);
 
 // Float32 kernel arguments of one state's LUT builds, derived on the host
 // exactly as the reference's update / prepare_frame round them

@@ -407,6 +407,20 @@ int32_t atmo_prepare_surface_lighting(AtmoHandle* h, const float* position_view,
 
 }  // extern "C"
 
+void composite_frame_constants(const atmo::Consts& k, CompositeArgs& a) {
+    // camera_ray (renderer.py:1850-1859): width / height and forward / thf
+    a.aspect = (float)a.width / (float)a.height;
+    a.fwd_t = atmo::f3{a.forward.x / a.thf, a.forward.y / a.thf, a.forward.z / a.thf};
+    // the sun disk's one-pixel filter (renderer.py:1930-1936)
+    a.pixel_filter = (float)atan2((double)(a.thf * 1.41421356f / (float)a.height), 1.0);
+    // sky_horizon_zenith at the sky-view altitude (renderer.py:482-490)
+    const float alt = fmaxf(a.sky_alt, 0.0f);
+    const float radius = a.bottom + alt;
+    const float rho = sqrtf(alt * (2.0f * a.bottom + alt));
+    const float tc = rho / fmaxf(radius, 1.0f);
+    a.sky_horizon = k.pi - (float)acos((double)fminf(1.0f, fmaxf(0.0f, tc)));
+}
+
 int32_t atmo_launch_composite_sky(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream) {
     ATMO_STAGE(h, launch_composite_sky, h->k, a, stream);
     return atmo_launch_status(h, "k_composite_sky");
@@ -463,6 +477,7 @@ int32_t atmo_composite(AtmoHandle* h, const float* scene_hdr, const float* space
     a.view = view;
     a.queue = h->queue;
     a.parity = h->queue_parity;
+    composite_frame_constants(h->k, a);
     if (view == ATMO_VIEW_COMPOSITE) h->queue_parity ^= 1;
     ATMO_STAGE(h, launch_composite, h->k, a, (hipStream_t)stream);
     return atmo_launch_status(h, "atmo_composite");

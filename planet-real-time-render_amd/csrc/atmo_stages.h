@@ -482,7 +482,7 @@ __global__ void __launch_bounds__(256) k_prepare(Consts k, long npix,
         f3 planet = camera + local;
         f3 radial = div3(planet, fmaxf(norm(planet), 1.0f));
         float sd = norm(local);
-        f3 local_ray = local / fmaxf(sd, 1.0f);
+        f3 local_ray = div3(local, fmaxf(sd, 1.0f));
         float sc = dot(radial, sun);
 #if ATMO_ROBUST
         if (cam_alt > 1.0e6f) {
@@ -511,21 +511,20 @@ __global__ void __launch_bounds__(256) k_prepare(Consts k, long npix,
 
 __device__ __forceinline__ f3 pixel_ray(const CompositeArgs& a, int px, int py, float& su,
                                         float& sv) {
-    float aspect = (float)a.width / (float)a.height;
-    su = ((float)px + 0.5f) / (float)a.width;
-    sv = ((float)py + 0.5f) / (float)a.height;
-    float sx = (su * 2.0f - 1.0f) * aspect;
-    float sy = sv * 2.0f - 1.0f;
-    return normalized((a.right * sx + a.up * sy) + a.forward / a.thf);
+    return camera_ray(px, py, a.width, a.height, a.aspect, a.right, a.up, a.fwd_t, su, sv);
+}
+__device__ __forceinline__ f3 pixel_ray(const CompositeArgs& a, long i) {
+    float su, sv;
+    return pixel_ray(a, (int)(i / a.height), (int)(i % a.height), su, sv);
 }
 
 // Sky branch of one COMPOSITE pixel (renderer.py:1882-1940).
 __device__ __forceinline__ void composite_sky_pixel(const Consts& k, const CompositeArgs& a,
                                                     long i) {
-    f3 ray = camera_ray((int)(i / a.height), (int)(i % a.height), a.width, a.height, a.right,
-                        a.up, a.forward, a.thf);
+    f3 ray = pixel_ray(a, i);
     f3 c = composite_sky(k, a.tlut, a.svlut, ray, ld3(a.scene, i), ld3(a.space, i), a.cam_alt,
-                         a.sky_alt, a.sun, a.disk, a.ar, a.thf, a.height, a.c0, a.c1, a.bottom);
+                         a.sky_alt, a.sun, a.disk, a.ar, a.pixel_filter, a.sky_horizon, a.c0, a.c1,
+                         a.bottom);
     st3(a.hdr, i, maxs(c, 0.0f));
 }
 
@@ -552,8 +551,7 @@ k_composite(Consts k, CompositeArgs a) {
     long j = 0;
     if ((int)threadIdx.x < count) {
         j = base + list[threadIdx.x];
-        f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height, a.right,
-                            a.up, a.forward, a.thf);
+        f3 ray = pixel_ray(a, j);
         float sd = norm(ld3(a.pos, j));
         f3 scat;
         if (surface_inscatter_t<false>(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun, a.irradiance,
@@ -583,8 +581,7 @@ __global__ void __launch_bounds__(256) k_composite_list(Consts k, CompositeArgs 
     if (blockIdx.x == 0 && threadIdx.x == 0) a.queue[1 - a.parity] = 0;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < count; e += gridDim.x * blockDim.x) {
         const long j = a.queue[2 + e];
-        f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height, a.right,
-                            a.up, a.forward, a.thf);
+        f3 ray = pixel_ray(a, j);
         float sd = norm(ld3(a.pos, j));
         f3 scat = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun, a.irradiance,
                                     a.bottom, a.ar);
@@ -605,7 +602,7 @@ __global__ void __launch_bounds__(256) k_composite_diagnostic(Consts k, Composit
     f3 color = mk(0.0f, 0.0f, 0.0f);
     switch (a.view) {
         case ATMO_VIEW_SKY_VIEW:
-            color = sample_sky_view(k, a.svlut, ray, a.sun, a.sky_alt, a.bottom);
+            color = sample_sky_view(k, a.svlut, ray, a.sun, a.sky_alt, a.bottom, a.sky_horizon);
             break;
         case ATMO_VIEW_CAMERA_TRANSMITTANCE:
             color = camera_transmittance(k, a.tlut, a.cam_alt, ray, a.bottom);

@@ -7,7 +7,9 @@
 // traffic counters (FETCH_SIZE, WRITE_SIZE) of the other kernels.
 // atmo_probe_arith_check compares the short correctly rounded square root,
 // reciprocal and quotient of the atmosphere kernels (atmo::csqrt, crcp,
-// cdiv) with the compiler's IEEE expansions over their stated domains.
+// cdiv) with the compiler's IEEE expansions over their stated domains, and
+// the kernels' float64 acos (atmo::acos_d) rounded to float32 with the
+// device library's acos rounded the same way.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -44,7 +46,8 @@ __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src,
 // every float with |b| in [2^-100, 2^101); counts[2]: cdiv != a / b over
 // hashed pairs with both exponents in [-60, 60] (2^32 pairs); counts[3]:
 // csqrt != sqrtf on (0, 2^-96), beyond 2^126 and for negative x, for
-// information.
+// information; counts[4]: float(acos_d(x)) != float(acos(x)) over every
+// float x in [-1, 1].
 __device__ __forceinline__ bool same_bits(float a, float b) {
     return __float_as_uint(a) == __float_as_uint(b) || (a != a && b != b);
 }
@@ -82,18 +85,29 @@ __global__ void __launch_bounds__(256) k_check_div(unsigned long long* counts, u
     if (!same_bits(atmo::cdiv(a, b), a / b)) atomicAdd(&counts[2], 1ull);
 }
 
+// bits in [0, 0x3f800000] (x in [0, 1]) with the given sign bit; one
+// atomic per wave with a mismatch
+__global__ void __launch_bounds__(256) k_check_acos(unsigned long long* counts, uint32_t base,
+                                                    uint32_t sign) {
+    const uint32_t i = base + blockIdx.x * blockDim.x + threadIdx.x;
+    const float x = __uint_as_float((i <= 0x3f800000u ? i : 0u) | sign);
+    const bool miss = i <= 0x3f800000u && !same_bits(atmo::cr_acos(x), (float)acos((double)x));
+    const unsigned long long m = __ballot(miss);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&counts[4], (unsigned long long)__popcll(m));
+}
+
 }  // namespace
 
 extern "C" {
 
-// Runs the three comparisons above on the null stream and copies the four
-// mismatch counts to counts[0..3].  Returns 0 on success, otherwise a
+// Runs the comparisons above on the null stream and copies the five
+// mismatch counts to counts[0..4].  Returns 0 on success, otherwise a
 // hipError_t value.
 int32_t atmo_probe_arith_check(uint64_t* counts) {
     unsigned long long* d = nullptr;
-    hipError_t e = hipMalloc(&d, 4 * sizeof(unsigned long long));
+    hipError_t e = hipMalloc(&d, 5 * sizeof(unsigned long long));
     if (e != hipSuccess) return (int32_t)e;
-    e = hipMemset(d, 0, 4 * sizeof(unsigned long long));
+    e = hipMemset(d, 0, 5 * sizeof(unsigned long long));
     // 2^32 sqrt inputs: 16 launches of 2^20 blocks x 256 threads
     for (uint32_t c = 0; c < 16 && e == hipSuccess; ++c) {
         hipLaunchKernelGGL(k_check_sqrt, dim3(1u << 20), dim3(256), 0, 0, d, c << 28);
@@ -108,12 +122,20 @@ int32_t atmo_probe_arith_check(uint64_t* counts) {
         hipLaunchKernelGGL(k_check_div, dim3(1u << 20), dim3(256), 0, 0, d, c << 28);
         e = hipGetLastError();
     }
+    // 0x3f800001 patterns per sign: 4 launches of 2^20 blocks x 256 threads cover [0, 2^30)
+    for (int sg = 0; sg < 2 && e == hipSuccess; ++sg) {
+        for (uint32_t c = 0; c < 4 && e == hipSuccess; ++c) {
+            hipLaunchKernelGGL(k_check_acos, dim3(1u << 20), dim3(256), 0, 0, d, c << 28,
+                               sg ? 0x80000000u : 0u);
+            e = hipGetLastError();
+        }
+    }
     if (e == hipSuccess) e = hipDeviceSynchronize();
-    unsigned long long h[4] = {0, 0, 0, 0};
+    unsigned long long h[5] = {0, 0, 0, 0, 0};
     if (e == hipSuccess) e = hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost);
     (void)hipFree(d);
     if (e != hipSuccess) return (int32_t)e;
-    for (int i = 0; i < 4; ++i) counts[i] = (uint64_t)h[i];
+    for (int i = 0; i < 5; ++i) counts[i] = (uint64_t)h[i];
     return 0;
 }
 

@@ -492,6 +492,8 @@ struct SurrogateArgs {
     float ar;
     f3 right, up, forward;
     float thf, c0, c1;
+    float aspect;  // width / height
+    f3 fwd_t;      // forward / thf
     const uint16_t* frag;
     const float* frag32;
     const float* bias;
@@ -533,7 +535,7 @@ __global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, Surrogate
         const bool has = (int)threadIdx.x < count;
         const long j = base + (has ? list[threadIdx.x] : 0);
         const f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height,
-                                  a.right, a.up, a.forward, a.thf);
+                                  a.aspect, a.right, a.up, a.fwd_t);
         const float sd = has ? norm(ld3(a.pos, j)) : 1.0f;
         const bool learned = has && in_domain(a.t, a.cam_alt, sd);
         // queue the out-of-domain surface pixels
@@ -907,6 +909,9 @@ int32_t atmo_composite_surrogate(AtmoHandle* h, AtmoMlp* m, const float* scene_h
     ca.view = ATMO_VIEW_COMPOSITE;
     ca.queue = h->queue;
     ca.parity = h->queue_parity;
+    composite_frame_constants(h->k, ca);
+    a.aspect = ca.aspect;
+    a.fwd_t = ca.fwd_t;
     h->queue_parity ^= 1;
     if (atmo_launch_composite_sky(h, ca, s) != ATMO_OK) return ATMO_EHIP;
     hipLaunchKernelGGL(k_composite_surrogate, dim3((unsigned)(groups < 1024 ? groups : 1024)),
