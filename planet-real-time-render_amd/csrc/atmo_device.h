@@ -199,6 +199,11 @@ __device__ __forceinline__ f3 clamp3(f3 a, float lo, float hi) {
 }
 __device__ __forceinline__ f3 exp3(f3 a) { return f3{cr_exp(a.x), cr_exp(a.y), cr_exp(a.z)}; }
 __device__ __forceinline__ f3 arr3(const float* a) { return f3{a[0], a[1], a[2]}; }
+// The texel at byte offset `off` of a LUT (offsets stay below 2^32).
+__device__ __forceinline__ f3 lut3(const float* __restrict__ base, uint32_t off) {
+    const float* p = reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + off);
+    return f3{p[0], p[1], p[2]};
+}
 __device__ __forceinline__ f3 ld3(const float* __restrict__ base, long i) {
     return f3{base[i * 3 + 0], base[i * 3 + 1], base[i * 3 + 2]};
 }
@@ -619,6 +624,9 @@ __device__ __forceinline__ float sun_cosine_to_u(float c, float h, float bottom)
 // ------------------------------------------------------------- sampling
 
 // Half-texel bilinear fetch (renderer.py:618-640, 655-683, 1503-1525).
+// Texels are addressed by 32-bit byte offsets from the LUT's base (every LUT
+// is far below 4 GiB), so each corner is one load with a scalar base and a
+// vector offset, and the four offsets share one multiply.
 __device__ __forceinline__ f3 bilinear(const float* __restrict__ lut, int w, int hgt, float u,
                                        float v) {
     float x = clampf(u * (float)w - 0.5f, 0.0f, (float)(w - 1));
@@ -629,8 +637,12 @@ __device__ __forceinline__ f3 bilinear(const float* __restrict__ lut, int w, int
     int y1 = min(y0 + 1, hgt - 1);
     float fx = x - (float)x0;
     float fy = y - (float)y0;
-    f3 low = ld3(lut, (long)x0 * hgt + y0) * (1.0f - fx) + ld3(lut, (long)x1 * hgt + y0) * fx;
-    f3 high = ld3(lut, (long)x0 * hgt + y1) * (1.0f - fx) + ld3(lut, (long)x1 * hgt + y1) * fx;
+    const uint32_t row = 12u * (uint32_t)hgt;
+    const uint32_t o00 = (uint32_t)x0 * row + 12u * (uint32_t)y0;
+    const uint32_t o10 = o00 + row * (uint32_t)(x1 - x0);
+    const uint32_t dy = 12u * (uint32_t)(y1 - y0);
+    f3 low = lut3(lut, o00) * (1.0f - fx) + lut3(lut, o10) * fx;
+    f3 high = lut3(lut, o00 + dy) * (1.0f - fx) + lut3(lut, o10 + dy) * fx;
     return low * (1.0f - fy) + high * fy;
 }
 
@@ -1079,11 +1091,15 @@ __device__ __forceinline__ f3 sample_aerial(const Consts& k, const float* __rest
     int x0 = (int)floorf(x), y0 = (int)floorf(y), z0 = (int)floorf(z);
     int x1 = min(x0 + 1, aw - 1), y1 = min(y0 + 1, ah - 1), z1 = min(z0 + 1, ad - 1);
     float fx = x - (float)x0, fy = y - (float)y0, fz = z - (float)z0;
-    auto at = [&](int X, int Y, int Z) { return ld3(ap, ((long)X * ah + Y) * ad + Z); };
-    f3 a = at(x0, y0, z0) * (1.0f - fx) + at(x1, y0, z0) * fx;
-    f3 b = at(x0, y1, z0) * (1.0f - fx) + at(x1, y1, z0) * fx;
-    f3 c = at(x0, y0, z1) * (1.0f - fx) + at(x1, y0, z1) * fx;
-    f3 e = at(x0, y1, z1) * (1.0f - fx) + at(x1, y1, z1) * fx;
+    // 32-bit byte offsets of the eight corners (the froxel volume is far below 4 GiB)
+    const uint32_t sz = 12u, sy = sz * (uint32_t)ad, sx = sy * (uint32_t)ah;
+    const uint32_t o = (uint32_t)x0 * sx + (uint32_t)y0 * sy + (uint32_t)z0 * sz;
+    const uint32_t dx = sx * (uint32_t)(x1 - x0), dy = sy * (uint32_t)(y1 - y0),
+                   dz = sz * (uint32_t)(z1 - z0);
+    f3 a = lut3(ap, o) * (1.0f - fx) + lut3(ap, o + dx) * fx;
+    f3 b = lut3(ap, o + dy) * (1.0f - fx) + lut3(ap, o + dx + dy) * fx;
+    f3 c = lut3(ap, o + dz) * (1.0f - fx) + lut3(ap, o + dx + dz) * fx;
+    f3 e = lut3(ap, o + dy + dz) * (1.0f - fx) + lut3(ap, o + dx + dy + dz) * fx;
     f3 low = a * (1.0f - fy) + b * fy;
     f3 high = c * (1.0f - fy) + e * fy;
     return low * (1.0f - fz) + high * fz;
