@@ -60,6 +60,17 @@ struct f3 {
     float x, y, z;
 };
 
+// a * b + c in one v_fma_f64 that reads c in place.  Left to itself the
+// compiler evaluates a polynomial step with the two-address v_fmac_f64,
+// whose addend is also its destination: with the coefficient c kept in a
+// register across loop iterations, every step then costs a v_mov_b64 copy
+// of c as well.  (Arithmetic only: no memory access, no wait.)
+__device__ __forceinline__ double fma_c(double a, double b, double c) {
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+
 // acos in float64 for |x| <= 1, within 1.2 ulp: asin(s) = s + s w P(w),
 // w = s^2 <= 1/4, with P the degree-12 Chebyshev fit of
 // (asin(sqrt(w)) - sqrt(w)) / w^1.5 on [0, 1/4] (fit error 1.5e-17);
@@ -70,18 +81,18 @@ struct f3 {
 // model does not need: its float64 results are only ever rounded to float32.
 __device__ __forceinline__ double asin_poly(double w) {
     double p = 0x1.d72b2bc8155f8p-6;
-    p = fma(p, w, -0x1.e6aaa8a0a04ccp-7);
-    p = fma(p, w, 0x1.1d189408314eep-6);
-    p = fma(p, w, 0x1.65a9c4dfcf8b2p-8);
-    p = fma(p, w, 0x1.52420b04b37bep-7);
-    p = fma(p, w, 0x1.782651caa6547p-7);
-    p = fma(p, w, 0x1.c9cf07674736ap-7);
-    p = fma(p, w, 0x1.1c4d35cf95421p-6);
-    p = fma(p, w, 0x1.6e8bb1c8209a2p-6);
-    p = fma(p, w, 0x1.f1c71c1db0623p-6);
-    p = fma(p, w, 0x1.6db6db6e31f13p-5);
-    p = fma(p, w, 0x1.3333333332ecap-4);
-    return fma(p, w, 0x1.5555555555556p-3);
+    p = fma_c(p, w, -0x1.e6aaa8a0a04ccp-7);
+    p = fma_c(p, w, 0x1.1d189408314eep-6);
+    p = fma_c(p, w, 0x1.65a9c4dfcf8b2p-8);
+    p = fma_c(p, w, 0x1.52420b04b37bep-7);
+    p = fma_c(p, w, 0x1.782651caa6547p-7);
+    p = fma_c(p, w, 0x1.c9cf07674736ap-7);
+    p = fma_c(p, w, 0x1.1c4d35cf95421p-6);
+    p = fma_c(p, w, 0x1.6e8bb1c8209a2p-6);
+    p = fma_c(p, w, 0x1.f1c71c1db0623p-6);
+    p = fma_c(p, w, 0x1.6db6db6e31f13p-5);
+    p = fma_c(p, w, 0x1.3333333332ecap-4);
+    return fma_c(p, w, 0x1.5555555555556p-3);
 }
 
 __device__ __forceinline__ double acos_d(double x) {
@@ -803,12 +814,12 @@ __device__ __forceinline__ float exp_neg_segment(float x) {
     if (x < 0.0625f) {
         const double t = -(double)x;
         double p = 1.0 / 362880.0;
-        p = fma(p, t, 1.0 / 40320.0);
-        p = fma(p, t, 1.0 / 5040.0);
-        p = fma(p, t, 1.0 / 720.0);
-        p = fma(p, t, 1.0 / 120.0);
-        p = fma(p, t, 1.0 / 24.0);
-        p = fma(p, t, 1.0 / 6.0);
+        p = fma_c(p, t, 1.0 / 40320.0);
+        p = fma_c(p, t, 1.0 / 5040.0);
+        p = fma_c(p, t, 1.0 / 720.0);
+        p = fma_c(p, t, 1.0 / 120.0);
+        p = fma_c(p, t, 1.0 / 24.0);
+        p = fma_c(p, t, 1.0 / 6.0);
         p = fma(p, t, 0.5);
         p = fma(p, t, 1.0);
         p = fma(p, t, 1.0);
