@@ -171,19 +171,27 @@ def probe_exp_rate() -> float | None:
 
 
 def time_frames(stages, steps: int, warmup: int, stream, device):
-    """Warm up, then time `steps` frames: wall ms/frame between two device
-    synchronisations, per-frame event durations and per-stage event means."""
+    """Warm up, then time `steps` frames back to back: wall ms/frame between
+    two device synchronisations.  A second pass of `steps` frames records
+    HIP events between the stages for the per-frame durations and per-stage
+    means (the event records themselves add a few microseconds per stage,
+    so they stay out of the wall-clock pass)."""
     import torch
 
     for _ in range(warmup):
         for _, fn in stages:
             fn()
     torch.cuda.synchronize(device)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for _, fn in stages:
+            fn()
+    torch.cuda.synchronize(device)
+    wall = (time.perf_counter() - t0) * 1e3 / steps
     n_st = len(stages)
     events = [[torch.cuda.Event(enable_timing=True) for _ in range(n_st + 1)]
               for _ in range(steps)]
     torch.cuda.synchronize(device)
-    t0 = time.perf_counter()
     for k in range(steps):
         ev = events[k]
         ev[0].record(stream)
@@ -191,7 +199,6 @@ def time_frames(stages, steps: int, warmup: int, stream, device):
             fn()
             ev[j + 1].record(stream)
     torch.cuda.synchronize(device)
-    wall = (time.perf_counter() - t0) * 1e3 / steps
     frame_ms = np.array([events[k][0].elapsed_time(events[k][n_st]) for k in range(steps)])
     stage_ms = {name: float(np.mean([events[k][j].elapsed_time(events[k][j + 1])
                                      for k in range(steps)]))

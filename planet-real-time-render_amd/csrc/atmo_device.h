@@ -487,19 +487,25 @@ __device__ __forceinline__ void transmittance_uv_to_ray(const Consts& k, float u
     distance = d;
 }
 
-// renderer.py:434-452
-__device__ __forceinline__ void transmittance_ray_to_uv(const Consts& k, float h, float c,
-                                                        float bottom, float& u, float& v) {
+// renderer.py:434-452, for an altitude h already clamped to [0, top] and
+// its rho = shell_rho(h, bottom).
+__device__ __forceinline__ void transmittance_uv_of(const Consts& k, float h, float rho, float c,
+                                                    float bottom, float& u, float& v) {
     float top = k.top;
-    h = clampf(h, 0.0f, top);
     float sh = shell_rho(top, bottom);
-    float rho = shell_rho(h, bottom);
     float d = distance_to_top(k, h, c, bottom);
     float dmin = top - h;
     float dmax = rho + sh;
     u = clampf(cdiv(d - dmin, fmaxf(dmax - dmin, 1.0e-6f)), 0.0f, 1.0f);
     const float shm = fmaxf(sh, 1.0e-6f);
     v = clampf(quot(rho, shm, crcp(shm)), 0.0f, 1.0f);
+}
+
+// renderer.py:434-452
+__device__ __forceinline__ void transmittance_ray_to_uv(const Consts& k, float h, float c,
+                                                        float bottom, float& u, float& v) {
+    h = clampf(h, 0.0f, k.top);
+    transmittance_uv_of(k, h, shell_rho(h, bottom), c, bottom, u, v);
 }
 
 // Warped interval boundaries (renderer.py:454-480).  The closest-approach
@@ -740,6 +746,7 @@ __device__ __forceinline__ float cos_of_rounded_acos(float x) {
 struct SunGeom {
     float visibility;
     float rc;   // cosine of the representative zenith
+    float rho;  // shell_rho(h, bottom), shared with the transmittance lookup
     bool fast;  // outside the penumbra band: visibility exactly 0 or 1
 };
 
@@ -747,11 +754,15 @@ __device__ __forceinline__ SunGeom sun_geometry(const Consts& k, float h, float 
                                                 float bottom, float angular_radius) {
     SunGeom g;
     float radius = bottom + fmaxf(h, 0.0f);
-    float hc = cdiv(-shell_rho(h, bottom), fmaxf(radius, 1.0f));
+    g.rho = shell_rho(h, bottom);
+    const float rr = crcp(fmaxf(radius, 1.0f));
+    float hc = quot(-g.rho, fmaxf(radius, 1.0f), rr);  // cdiv(-rho, max(radius, 1))
     {
         const float dl = 1.05f * angular_radius;
         const float cd = 1.0f - 0.5f * dl * dl, sd = dl;  // cos d, sin d to O(d^3)
-        const float hs = csqrt(fmaxf(1.0f - hc * hc, 0.0f));
+        // the horizon sine sqrt(1 - hc^2) is bottom / radius, since
+        // rho^2 = radius^2 - bottom^2; it only places the band edges
+        const float hs = bottom * rr;
         const float up = hc * cd + hs * sd, down = hc * cd - hs * sd;
         if (sun_cosine <= down) {
             g.fast = true;
@@ -790,7 +801,13 @@ __device__ __forceinline__ f3 solar_transmittance(const Consts& k, const float* 
                                                   const SunGeom& g, float h, float bottom,
                                                   float angular_radius) {
     if (g.visibility == 0.0f) return mk(0.0f, 0.0f, 0.0f);
-    f3 t = sample_transmittance(k, tlut, h, g.rc, bottom);
+    // sample_transmittance(h, rc): shell_rho of clamp(max(h, 0), 0, top) is
+    // g.rho below the top and shell_rho(top) above it
+    const float hm = fmaxf(h, 0.0f);
+    const float rho = hm <= k.top ? g.rho : shell_rho(k.top, bottom);
+    float u, v;
+    transmittance_uv_of(k, fminf(k.top, hm), rho, g.rc, bottom, u, v);
+    f3 t = bilinear(tlut, k.tw, k.th, u, v);
     return g.fast ? t : t * g.visibility;
 }
 
