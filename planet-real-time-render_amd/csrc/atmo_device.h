@@ -117,6 +117,13 @@ __device__ __forceinline__ float cr_exp(float x) { return (float)exp((double)x);
 __device__ __forceinline__ float cr_acos(float x) { return (float)acos_d((double)x); }
 __device__ __forceinline__ float cr_cos(float x) { return (float)cos((double)x); }
 __device__ __forceinline__ float cr_sin(float x) { return (float)sin((double)x); }
+// cr_sin and cr_cos of one argument with one float64 range reduction.
+__device__ __forceinline__ void cr_sincos(float x, float& s, float& c) {
+    double sd, cd;
+    sincos((double)x, &sd, &cd);
+    s = (float)sd;
+    c = (float)cd;
+}
 // x^1.5 as x sqrt(x) in float64, rounded once: like a float64 pow rounded
 // to float32, it is the correctly rounded value unless x^1.5 lies within a
 // few float64 ulps of a float32 rounding boundary.
@@ -1052,7 +1059,8 @@ __device__ __forceinline__ f3 surface_transmittance(const Consts& k,
         float surface_alt = altitude_on_ray(h0, ray.y, end, bottom);
         f3 s2t = sample_transmittance(k, tlut, surface_alt, dot(surface_radial, reverse), bottom);
         f3 n2t = sample_transmittance(k, tlut, near_alt, dot(near_radial, reverse), bottom);
-        t = clamp3(s2t / maxs(n2t, 1.0e-6f), 0.0f, 1.0f);
+        const f3 q = maxs(n2t, 1.0e-6f);
+        t = clamp3(mk(cdiv(s2t.x, q.x), cdiv(s2t.y, q.y), cdiv(s2t.z, q.z)), 0.0f, 1.0f);
     }
     return t;
 }
@@ -1086,12 +1094,12 @@ __device__ __forceinline__ f3 sample_sky_view(const Consts& k, const float* __re
                                               f3 ray, f3 sun, float h, float bottom,
                                               float horizon) {
     float zenith = cr_acos(clampf(ray.y, -1.0f, 1.0f));
-    float vl = sqrtf(ray.x * ray.x + ray.z * ray.z);
-    float sl = sqrtf(sun.x * sun.x + sun.z * sun.z);
+    float vl = csqrt(ray.x * ray.x + ray.z * ray.z);
+    float sl = csqrt(sun.x * sun.x + sun.z * sun.z);
     float rel = 0.0f;
     if (vl > 1.0e-6f && sl > 1.0e-6f)
-        rel = cr_acos(clampf((ray.x * sun.x + ray.z * sun.z) / (vl * sl), -1.0f, 1.0f));
-    float u = rel / k.pi;
+        rel = cr_acos(clampf(cdiv(ray.x * sun.x + ray.z * sun.z, vl * sl), -1.0f, 1.0f));
+    float u = cdiv(rel, k.pi);
 #if ATMO_ROBUST
     (void)zenith;
     (void)horizon;
@@ -1148,12 +1156,12 @@ __device__ __forceinline__ f3 composite_sky(const Consts& k, const float* __rest
     f3 sky = sample_sky_view(k, svlut, ray, sun, sky_alt, bottom, horizon);
     f3 star = space * trans;
     const f3 lum = mk(0.2126f, 0.7152f, 0.0722f);
-    float contrast = dot(star, lum) / fmaxf(dot(sky, lum), 1.0e-5f);
-    float vis = clampf((contrast - c0) / fmaxf(c1 - c0, 1.0e-5f), 0.0f, 1.0f);
+    float contrast = cdiv(dot(star, lum), fmaxf(dot(sky, lum), 1.0e-5f));
+    float vis = clampf(cdiv(contrast - c0, fmaxf(c1 - c0, 1.0e-5f)), 0.0f, 1.0f);
     vis = vis * vis * (3.0f - 2.0f * vis);
     color = (sky + color * trans) + star * vis;
     float angular_distance = cr_acos(clampf(dot(ray, sun), -1.0f, 1.0f));
-    float cov = clampf((ar + pixel_filter - angular_distance) / fmaxf(2.0f * pixel_filter, 1.0e-8f),
+    float cov = clampf(cdiv(ar + pixel_filter - angular_distance, fmaxf(2.0f * pixel_filter, 1.0e-8f)),
                        0.0f, 1.0f);
     cov = cov * cov * (3.0f - 2.0f * cov);
     return color + (disk * trans) * cov;

@@ -180,7 +180,9 @@ __device__ __forceinline__ void multi_scattering_body(const Consts& k, float bot
             float dc = 1.0f - 2.0f * si / (float)N;
             float ds = sqrtf(fmaxf(1.0f - dc * dc, 0.0f));
             float az = (float)di * 2.39996323f;
-            ray = mk(ds * cr_cos(az), dc, ds * cr_sin(az));
+            float saz, caz;
+            cr_sincos(az, saz, caz);
+            ray = mk(ds * caz, dc, ds * saz);
             float start;
             segment_vec(k, origin, ray, bottom, start, end, valid, ground);
             if (valid && s0 < s1) {
@@ -278,10 +280,12 @@ __device__ __forceinline__ void sky_view_body(const Consts& k, float bottom, flo
     float vsn = (float)sin(zenith_w);
 #else
     float zenith = sky_v_to_zenith(k, view_v, sky_horizon_zenith(k, safe_alt, bottom));
-    float vc = cr_cos(zenith);
-    float vsn = cr_sin(zenith);
+    float vc, vsn;
+    cr_sincos(zenith, vsn, vc);
 #endif
-    f3 ray = mk(vsn * cr_cos(az), vc, vsn * cr_sin(az));
+    float saz, caz;
+    cr_sincos(az, saz, caz);
+    f3 ray = mk(vsn * caz, vc, vsn * saz);
     float start, end;
     bool valid, ground;
     segment_from_altitude(k, safe_alt, vc, bottom, start, end, valid, ground);
@@ -314,8 +318,12 @@ __device__ __forceinline__ void sky_view_body(const Consts& k, float bottom, flo
 }
 
 // _build_aerial_perspective (renderer.py:1304-1420).  One wave64 per froxel
-// ray, lane = depth slice; slices beyond 64 are processed in chunks with the
-// previous chunk's composition carried in.
+// ray.  P lanes per depth slice (P = 4 for up to 16 slices, 2 for up to 32,
+// else 1), each marching a contiguous part of the slice's steps, so a
+// shallow froxel volume still fills the wave; one ordered (L, T) scan over
+// the wave composes the parts and slices, and the last part of each slice
+// stores its cumulative value.  Slices beyond a wave are processed in
+// chunks with the previous chunk's composition carried in.
 __device__ __forceinline__ void aerial_body(const Consts& k, int width, int height, float bottom,
                                             float cam_alt, f3 sun, f3 irradiance, float ar, f3 right,
                                             f3 up, f3 forward, float thf,
@@ -341,9 +349,13 @@ __device__ __forceinline__ void aerial_body(const Consts& k, int width, int heig
     const float neg_od = -dot(origin, ray);
     const int D = k.ad;
     const int spp = k.ap_spp;
+    const int P = D <= 16 ? 4 : (D <= 32 ? 2 : 1);
+    const int part = lane % P;
+    const int per = (spp + P - 1) / P;
+    const int s0 = min(part * per, spp), s1 = min(s0 + per, spp);
     f3 carry_l = mk(0.0f, 0.0f, 0.0f), carry_t = mk(1.0f, 1.0f, 1.0f);
-    for (int base = 0; base < D; base += 64) {
-        const int di = base + lane;
+    for (int base = 0; base < D; base += 64 / P) {
+        const int di = base + lane / P;
         f3 L = mk(0.0f, 0.0f, 0.0f), T = mk(1.0f, 1.0f, 1.0f);
         if (valid && di < D) {
             float previous = di == 0 ? a_start
@@ -353,8 +365,8 @@ __device__ __forceinline__ void aerial_body(const Consts& k, int width, int heig
             float ie = fminf(target, a_end);
             if (ie > is) {
                 Warp w = make_warp(neg_od, is, ie);
-                float d0 = boundary(w, 0.0f / (float)spp);
-                for (int s = 0; s < spp; ++s) {
+                float d0 = boundary(w, (float)s0 / (float)spp);
+                for (int s = s0; s < s1; ++s) {
                     float d1 = boundary(w, (float)(s + 1) / (float)spp);
                     float dist = (d0 + d1) * 0.5f;
                     float step = d1 - d0;
@@ -374,7 +386,7 @@ __device__ __forceinline__ void aerial_body(const Consts& k, int width, int heig
         scan_segments<64>(L, T, lane);
         f3 cl = carry_l + carry_t * L;
         f3 ct = carry_t * T;
-        if (di < D) st3(ap, wave * D + di, maxs(cl, 0.0f));
+        if (di < D && part == P - 1) st3(ap, wave * D + di, maxs(cl, 0.0f));
         carry_l = mk(__shfl(cl.x, 63, 64), __shfl(cl.y, 63, 64), __shfl(cl.z, 63, 64));
         carry_t = mk(__shfl(ct.x, 63, 64), __shfl(ct.y, 63, 64), __shfl(ct.z, 63, 64));
     }

@@ -47,7 +47,8 @@ __global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src,
 // hashed pairs with both exponents in [-60, 60] (2^32 pairs); counts[3]:
 // csqrt != sqrtf on (0, 2^-96), beyond 2^126 and for negative x, for
 // information; counts[4]: float(acos_d(x)) != float(acos(x)) over every
-// float x in [-1, 1].
+// float x in [-1, 1]; counts[5]: cr_sincos(x) != (cr_sin(x), cr_cos(x))
+// over every float x in [0, 256).
 __device__ __forceinline__ bool same_bits(float a, float b) {
     return __float_as_uint(a) == __float_as_uint(b) || (a != a && b != b);
 }
@@ -96,18 +97,30 @@ __global__ void __launch_bounds__(256) k_check_acos(unsigned long long* counts, 
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&counts[4], (unsigned long long)__popcll(m));
 }
 
+// every float in [0, 256): bit patterns [0, 0x43800000)
+__global__ void __launch_bounds__(256) k_check_sincos(unsigned long long* counts, uint32_t base) {
+    const uint32_t i = base + blockIdx.x * blockDim.x + threadIdx.x;
+    const float x = __uint_as_float(i < 0x43800000u ? i : 0u);
+    float s, c;
+    atmo::cr_sincos(x, s, c);
+    const bool miss = i < 0x43800000u &&
+                      !(same_bits(s, atmo::cr_sin(x)) && same_bits(c, atmo::cr_cos(x)));
+    const unsigned long long m = __ballot(miss);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&counts[5], (unsigned long long)__popcll(m));
+}
+
 }  // namespace
 
 extern "C" {
 
-// Runs the comparisons above on the null stream and copies the five
-// mismatch counts to counts[0..4].  Returns 0 on success, otherwise a
+// Runs the comparisons above on the null stream and copies the six
+// mismatch counts to counts[0..5].  Returns 0 on success, otherwise a
 // hipError_t value.
 int32_t atmo_probe_arith_check(uint64_t* counts) {
     unsigned long long* d = nullptr;
-    hipError_t e = hipMalloc(&d, 5 * sizeof(unsigned long long));
+    hipError_t e = hipMalloc(&d, 6 * sizeof(unsigned long long));
     if (e != hipSuccess) return (int32_t)e;
-    e = hipMemset(d, 0, 5 * sizeof(unsigned long long));
+    e = hipMemset(d, 0, 6 * sizeof(unsigned long long));
     // 2^32 sqrt inputs: 16 launches of 2^20 blocks x 256 threads
     for (uint32_t c = 0; c < 16 && e == hipSuccess; ++c) {
         hipLaunchKernelGGL(k_check_sqrt, dim3(1u << 20), dim3(256), 0, 0, d, c << 28);
@@ -130,12 +143,17 @@ int32_t atmo_probe_arith_check(uint64_t* counts) {
             e = hipGetLastError();
         }
     }
+    // 0x43800000 patterns: 5 launches of 2^28 threads cover [0, 5 * 2^28)
+    for (uint32_t c = 0; c < 5 && e == hipSuccess; ++c) {
+        hipLaunchKernelGGL(k_check_sincos, dim3(1u << 20), dim3(256), 0, 0, d, c << 28);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipDeviceSynchronize();
-    unsigned long long h[5] = {0, 0, 0, 0, 0};
+    unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
     if (e == hipSuccess) e = hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost);
     (void)hipFree(d);
     if (e != hipSuccess) return (int32_t)e;
-    for (int i = 0; i < 5; ++i) counts[i] = (uint64_t)h[i];
+    for (int i = 0; i < 6; ++i) counts[i] = (uint64_t)h[i];
     return 0;
 }
 
