@@ -665,8 +665,10 @@ __device__ __forceinline__ f3 bilinear(const float* __restrict__ lut, int w, int
     const uint32_t o00 = (uint32_t)x0 * row + 12u * (uint32_t)y0;
     const uint32_t o10 = o00 + row * (uint32_t)(x1 - x0);
     const uint32_t dy = 12u * (uint32_t)(y1 - y0);
-    f3 low = lut3(lut, o00) * (1.0f - fx) + lut3(lut, o10) * fx;
-    f3 high = lut3(lut, o00 + dy) * (1.0f - fx) + lut3(lut, o10 + dy) * fx;
+    const f3 c00 = lut3(lut, o00), c10 = lut3(lut, o10);
+    const f3 c01 = lut3(lut, o00 + dy), c11 = lut3(lut, o10 + dy);
+    f3 low = c00 * (1.0f - fx) + c10 * fx;
+    f3 high = c01 * (1.0f - fx) + c11 * fx;
     return low * (1.0f - fy) + high * fy;
 }
 
