@@ -53,6 +53,9 @@ struct Consts {
     float mie_a, mie_b, mie_c, mie_d;
     float r_h_r, r_h_m, r_half_width, r_top;  // 1.0f / (h_r, h_m, half_width, top), rounded once
     int tw, th, mw, mh, sw, sh, aw, ah, ad;
+    // the bilinear LUTs' sizes as floats, and their largest texel index
+    float tw_f, th_f, mw_f, mh_f, sw_f, sh_f;
+    float tw1_f, th1_f, mw1_f, mh1_f, sw1_f, sh1_f;
     int t_steps, ms_dirs, ms_steps, sv_steps, ap_spp, hz_steps, term_sub;
 };
 
@@ -651,12 +654,13 @@ __device__ __forceinline__ float sun_cosine_to_u(float c, float h, float bottom)
 // Texels are addressed by 32-bit byte offsets from the LUT's base (every LUT
 // is far below 4 GiB), so each corner is one load with a scalar base and a
 // vector offset, and the four offsets share one multiply.
-__device__ __forceinline__ f3 bilinear(const float* __restrict__ lut, int w, int hgt, float u,
-                                       float v) {
-    float x = clampf(u * (float)w - 0.5f, 0.0f, (float)(w - 1));
-    float y = clampf(v * (float)hgt - 0.5f, 0.0f, (float)(hgt - 1));
-    int x0 = (int)floorf(x);
-    int y0 = (int)floorf(y);
+__device__ __forceinline__ f3 bilinear(const float* __restrict__ lut, int w, int hgt, float wf,
+                                       float hf, float w1f, float h1f, float u, float v) {
+    float x = clampf(u * wf - 0.5f, 0.0f, w1f);
+    float y = clampf(v * hf - 0.5f, 0.0f, h1f);
+    // x, y >= 0: truncation is floor
+    int x0 = (int)x;
+    int y0 = (int)y;
     int x1 = min(x0 + 1, w - 1);
     int y1 = min(y0 + 1, hgt - 1);
     float fx = x - (float)x0;
@@ -677,7 +681,7 @@ __device__ __forceinline__ f3 sample_transmittance(const Consts& k, const float*
                                                    float h, float c, float bottom) {
     float u, v;
     transmittance_ray_to_uv(k, fmaxf(h, 0.0f), c, bottom, u, v);
-    return bilinear(tlut, k.tw, k.th, u, v);
+    return bilinear(tlut, k.tw, k.th, k.tw_f, k.th_f, k.tw1_f, k.th1_f, u, v);
 }
 
 // renderer.py:642-683
@@ -687,7 +691,7 @@ __device__ __forceinline__ f3 sample_multi_scattering(const Consts& k,
     float af = clampf(quot(h, k.top, k.r_top), 0.0f, 1.0f);
     float u = sun_cosine_to_u(sun_cosine, fmaxf(h, 0.0f), bottom);
     float v = csqrt(af);
-    return bilinear(mslut, k.mw, k.mh, u, v);
+    return bilinear(mslut, k.mw, k.mh, k.mw_f, k.mh_f, k.mw1_f, k.mh1_f, u, v);
 }
 
 // ------------------------------------------------------- sun and phases
@@ -816,7 +820,7 @@ __device__ __forceinline__ f3 solar_transmittance(const Consts& k, const float* 
     const float rho = hm <= k.top ? g.rho : shell_rho(k.top, bottom);
     float u, v;
     transmittance_uv_of(k, fminf(k.top, hm), rho, g.rc, bottom, u, v);
-    f3 t = bilinear(tlut, k.tw, k.th, u, v);
+    f3 t = bilinear(tlut, k.tw, k.th, k.tw_f, k.th_f, k.tw1_f, k.th1_f, u, v);
     return g.fast ? t : t * g.visibility;
 }
 
@@ -1110,7 +1114,7 @@ __device__ __forceinline__ f3 sample_sky_view(const Consts& k, const float* __re
     (void)h;
     float v = sky_zenith_to_v(k, zenith, horizon);
 #endif
-    return bilinear(svlut, k.sw, k.sh, u, v);
+    return bilinear(svlut, k.sw, k.sh, k.sw_f, k.sh_f, k.sw1_f, k.sh1_f, u, v);
 }
 
 // renderer.py:1527-1601

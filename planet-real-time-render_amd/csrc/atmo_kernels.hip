@@ -139,6 +139,18 @@ Consts make_consts(const AtmoParams& p) {
     k.mh = p.multi_scattering_lut_height;
     k.sw = p.sky_view_lut_width;
     k.sh = p.sky_view_lut_height;
+    k.tw_f = (float)k.tw;
+    k.th_f = (float)k.th;
+    k.mw_f = (float)k.mw;
+    k.mh_f = (float)k.mh;
+    k.sw_f = (float)k.sw;
+    k.sh_f = (float)k.sh;
+    k.tw1_f = (float)(k.tw - 1);
+    k.th1_f = (float)(k.th - 1);
+    k.mw1_f = (float)(k.mw - 1);
+    k.mh1_f = (float)(k.mh - 1);
+    k.sw1_f = (float)(k.sw - 1);
+    k.sh1_f = (float)(k.sh - 1);
     k.aw = p.aerial_lut_width;
     k.ah = p.aerial_lut_height;
     k.ad = p.aerial_lut_depth;
