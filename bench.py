@@ -83,12 +83,13 @@ class PlainFrame:
     """One cold frame through the facade's kernel methods, with the argument
     derivation of the reference's update / prepare_frame / composite."""
 
-    def __init__(self, pkg, inputs, device):
+    def __init__(self, pkg, inputs, device, precision: str = "reference"):
         import torch
 
         self.torch = torch
         self.inputs = inputs
-        self.r = pkg.AtmosphereRenderer(inputs.width, inputs.height, inputs.config, device)
+        self.r = pkg.AtmosphereRenderer(inputs.width, inputs.height, inputs.config, device,
+                                        precision=precision)
         to_dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
         self.pos = to_dev(inputs.position_view)
         self.sid = to_dev(inputs.surface_id)
@@ -307,6 +308,18 @@ def main() -> int:
     else:
         log(f"no surrogate weights at {weights}.bin: surrogate frame not measured")
 
+    # The same frame in the robust precision (float64 islands where float32
+    # cancels at planet scale): its cost, reported beside the plain frame.
+    rframe = PlainFrame(pkg, inputs, device, precision="robust")
+    rframe.r.autotune(inputs.planet_radius_m, inputs.camera_radius_m, inputs.sun_local,
+                      inputs.lighting.solar_irradiance, inputs.lighting.sun_angular_radius_degrees)
+    r_wall, _, r_stage_ms = time_frames(rframe.stages(), max(args.steps // 4, 5), args.warmup,
+                                        stream, device)
+    robust_hdr = rframe.hdr.cpu().numpy()
+    rframe.r.close()
+    log(f"robust precision: {r_wall:.4f} ms/frame (wall), composite "
+        f"{r_stage_ms['composite'] * 1e3:.2f} us")
+
     batch = None if args.no_batch else bench_batch(pkg, scene, device, max(args.steps // 5, 5))
     if batch:
         log(f"batch: {batch['states']} states in {batch['ms_per_batch']:.3f} ms "
@@ -327,7 +340,8 @@ def main() -> int:
         from oracle.ref_cpu import RefCpu
 
         ref64 = RefCpu("f64").frame(inputs)["hdr"]
-        accuracy = {"plain_vs_f64": rel_err_stats(plain_hdr, ref64)}
+        accuracy = {"plain_vs_f64": rel_err_stats(plain_hdr, ref64),
+                    "robust_vs_f64": rel_err_stats(robust_hdr, ref64)}
         if surrogate is not None:
             accuracy["surrogate_vs_f64"] = rel_err_stats(surrogate["hdr"], ref64)
             accuracy["surrogate_vs_f64_surface_pixels"] = rel_err_stats(
@@ -429,6 +443,8 @@ def main() -> int:
         "camera_altitude_m": round(inputs.camera_altitude_m, 3),
         "sun_zenith_cos": round(float(inputs.sun_local[1]), 5),
         "parallelism": "single GPU",
+        "robust_ms": round(r_wall, 4),
+        "robust_stage_us": {k: round(v * 1e3, 2) for k, v in r_stage_ms.items()},
     }
     if surrogate is not None:
         config.update({
