@@ -858,16 +858,45 @@ __device__ __forceinline__ float exp_neg_segment(float x) {
     return expf(-x);
 }
 
+#if ATMO_ROBUST
+// One channel of the robust segment: T = e^-x exactly as exp_neg_segment,
+// and (1 - e^-x) / sigma from 1 - e^-x in float64, rounded once.  For
+// x < 1/16 both come from one polynomial: q = (1 - e^-x) / x is the Taylor
+// sum whose last Horner step gives exp_neg_segment's e^-x = 1 - x q, so T
+// keeps its bits and 1 - e^-x = x q has no cancellation.  Above 1/16,
+// 1 - exp(-x) in float64 is within 16 float64 ulps, far below float32.
+__device__ __forceinline__ void robust_segment(float x, float sigma, float step, float& t_out,
+                                               float& i_out) {
+    if (x < 0.0625f) {
+        const double t = -(double)x;
+        double q = 1.0 / 362880.0;
+        q = fma_c(q, t, 1.0 / 40320.0);
+        q = fma_c(q, t, 1.0 / 5040.0);
+        q = fma_c(q, t, 1.0 / 720.0);
+        q = fma_c(q, t, 1.0 / 120.0);
+        q = fma_c(q, t, 1.0 / 24.0);
+        q = fma_c(q, t, 1.0 / 6.0);
+        q = fma(q, t, 0.5);
+        q = fma(q, t, 1.0);
+        t_out = (float)fma(q, t, 1.0);
+        i_out = sigma > 1.0e-12f ? cdiv((float)((double)x * q), sigma) : step;
+        return;
+    }
+    t_out = expf(-x);
+    i_out = sigma > 1.0e-12f ? cdiv((float)(1.0 - exp(-(double)x)), sigma) : step;
+}
+#endif
+
 // renderer.py:806-816
 __device__ __forceinline__ void segment_integral(f3 ext, float step, f3& trans, f3& integ) {
     f3 x = mins(ext * step, 80.0f);
-    trans = f3{exp_neg_segment(x.x), exp_neg_segment(x.y), exp_neg_segment(x.z)};
 #if ATMO_ROBUST
-    // 1 - exp(-x) cancels for small x: -expm1(-x) in float64, rounded once
-    integ.x = ext.x > 1.0e-12f ? (float)(-expm1(-(double)x.x)) / ext.x : step;
-    integ.y = ext.y > 1.0e-12f ? (float)(-expm1(-(double)x.y)) / ext.y : step;
-    integ.z = ext.z > 1.0e-12f ? (float)(-expm1(-(double)x.z)) / ext.z : step;
+    // 1 - exp(-x) cancels for small x: it is formed in float64, rounded once
+    robust_segment(x.x, ext.x, step, trans.x, integ.x);
+    robust_segment(x.y, ext.y, step, trans.y, integ.y);
+    robust_segment(x.z, ext.z, step, trans.z, integ.z);
 #else
+    trans = f3{exp_neg_segment(x.x), exp_neg_segment(x.y), exp_neg_segment(x.z)};
     // (1 - T) * rcp(sigma): within 2 ulp of the quotient, not amplified
     integ.x = ext.x > 1.0e-12f ? (1.0f - trans.x) * __builtin_amdgcn_rcpf(ext.x) : step;
     integ.y = ext.y > 1.0e-12f ? (1.0f - trans.y) * __builtin_amdgcn_rcpf(ext.y) : step;
