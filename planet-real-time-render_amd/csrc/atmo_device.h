@@ -116,6 +116,28 @@ __device__ __forceinline__ double acos_d(double x) {
     return x > 0.0 ? 2.0 * q : (pi_hi - 2.0 * q) + pi_lo;
 }
 
+// float64 square root and quotient for the robust precision's float64
+// islands, from float32 seeds: v_rsq_f32 / v_rcp_f32 (2^-23), two Newton
+// corrections in float64 (to ~2^-69 relative) and, for the quotient, one
+// Markstein correction.  Within an ulp of the float64 result, so their
+// float32 roundings are the correctly rounded ones except within ~2^-52 of
+// a float32 rounding boundary.  Domain: x = 0 or x in [2^-96, 2^126), and
+// |b| in the float32 normal range (squared lengths and radii of the model).
+__device__ __forceinline__ double dsqrt(double x) {
+    const double y = (double)__builtin_amdgcn_rsqf(fmaxf((float)x, 0x1p-96f));
+    const double h = 0.5 * y;
+    double g = x * y;
+    g = fma(fma(-g, g, x), h, g);
+    return fma(fma(-g, g, x), h, g);
+}
+__device__ __forceinline__ double ddiv(double a, double b) {
+    double y = (double)__builtin_amdgcn_rcpf((float)b);
+    y = fma(fma(-b, y, 1.0), y, y);
+    y = fma(fma(-b, y, 1.0), y, y);
+    const double q = a * y;
+    return fma(fma(-q, b, a), y, q);
+}
+
 __device__ __forceinline__ float cr_exp(float x) { return (float)exp((double)x); }
 __device__ __forceinline__ float cr_acos(float x) { return (float)acos_d((double)x); }
 __device__ __forceinline__ float cr_cos(float x) { return (float)cos((double)x); }
@@ -289,13 +311,13 @@ __device__ __forceinline__ void sphere_roots(const Consts& k, f3 origin, f3 ray,
         const double rx = ray.x, ry = ray.y, rz = ray.z;
         const double pr = (ox * rx + oy * ry) + oz * rz;
         const double px = ox - rx * pr, py = oy - ry * pr, pz = oz - rz * pr;
-        const double perp = sqrt((px * px + py * py) + pz * pz);
+        const double perp = dsqrt((px * px + py * py) + pz * pz);
         const double clear = (double)radius - perp;
         hit = clear >= 0.0;
         near = k.large;
         far = -k.large;
         if (hit) {
-            const double root = sqrt(fmax(fmax(clear, 0.0) * ((double)radius + perp), 0.0));
+            const double root = dsqrt(fmax(fmax(clear, 0.0) * ((double)radius + perp), 0.0));
             near = (float)(-pr - root);
             far = (float)(-pr + root);
         }
@@ -329,8 +351,8 @@ __device__ __forceinline__ float altitude_on_ray(float h0, float c, float t, flo
         // float64 island: the radial delta cancels near tangency
         const double a = fmax((double)h0, 0.0), b = bottom, cc = c, tt = t;
         const double delta = a * (2.0 * b + a) + 2.0 * (b + a) * cc * tt + tt * tt;
-        const double len = sqrt(fmax(b * b + delta, 0.0));
-        return (float)(delta / fmax(len + b, 1.0));
+        const double len = dsqrt(fmax(b * b + delta, 0.0));
+        return (float)ddiv(delta, fmax(len + b, 1.0));
     }
 #endif
     float a = fmaxf(h0, 0.0f);
@@ -354,12 +376,12 @@ __device__ __forceinline__ void shell_roots(const Consts& k, float h0, float c, 
         near = k.large;
         far = -k.large;
         if (hit) {
-            const double root = sqrt(fmax(disc, 0.0));
+            const double root = dsqrt(fmax(disc, 0.0));
             const double q = -hl - (hl < 0.0 ? -1.0 : 1.0) * root;
             double r0 = -hl - root, r1 = -hl + root;
             if (fabs(q) > (double)1.0e-12f) {
                 r0 = q;
-                r1 = constant / q;
+                r1 = ddiv(constant, q);
             }
             near = (float)fmin(r0, r1);
             far = (float)fmax(r0, r1);
@@ -461,7 +483,7 @@ __device__ __forceinline__ float distance_to_top(const Consts& k, float h, float
         const double radius = (double)bottom + fmax((double)h, 0.0), cc = c, top = k.top;
         const double delta = (top - (double)h) * (2.0 * (double)bottom + top + (double)h);
         const double disc = radius * radius * cc * cc + delta;
-        return (float)fmax(-radius * cc + sqrt(fmax(disc, 0.0)), 0.0);
+        return (float)fmax(-radius * cc + dsqrt(fmax(disc, 0.0)), 0.0);
     }
 #endif
     float radius = bottom + fmaxf(h, 0.0f);
