@@ -399,7 +399,9 @@ __global__ void __launch_bounds__(256) k_transmittance(Consts k, float bottom, f
     transmittance_body<G>(k, bottom, tlut, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-__global__ void __launch_bounds__(256) k_multi_scattering(Consts k, float bottom, f3 irradiance,
+// Three waves per SIMD, as for the batched build (28.6 against 30.0 us at 1024 texels).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
+k_multi_scattering(Consts k, float bottom, f3 irradiance,
                                                           float ar, const float* __restrict__ tlut,
                                                           float* __restrict__ mslut) {
     multi_scattering_body(k, bottom, irradiance, ar, tlut, mslut, (long)blockIdx.x);
