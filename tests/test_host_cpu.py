@@ -268,3 +268,46 @@ def test_markstein_quotient_equals_ieee_division():
         q1 = (q.astype(np.float64) + rem.astype(np.float64) * np.float64(rb)).astype(np.float32)
         want = (a / b).astype(np.float32)
         assert np.array_equal(q1, want), (float(b), int(np.sum(q1 != want)))
+
+
+def test_acos_polynomial_table():
+    """The coefficients of the device's float64 acos (asin_poly in
+    atmo_device.h, read from the source) reproduce acos to ~1 float64 ulp:
+    the same evaluation (Horner in float64, the sqrt branch for |x| >= 1/2,
+    the pi/2 and pi splits) emulated with numpy against math.acos on 2e5
+    float32 arguments.  A wrong or mistyped coefficient fails by orders of
+    magnitude more."""
+    src = (REPO / "planet-real-time-render_amd" / "csrc" / "atmo_device.h").read_text()
+    body = src[src.index("double asin_poly(double w) {"):]
+    body = body[:body.index("}")]
+    coef = [float.fromhex(h) for h in re.findall(r"(-?0x[0-9a-f.]+p[-+]?\d+)", body)]
+    assert len(coef) == 13
+    pio2_hi, pio2_lo = float.fromhex("0x1.921fb54442d18p+0"), float.fromhex("0x1.1a62633145c07p-54")
+    pi_hi, pi_lo = float.fromhex("0x1.921fb54442d18p+1"), float.fromhex("0x1.1a62633145c07p-53")
+
+    def poly(w):
+        p = np.full_like(w, coef[0])
+        for c in coef[1:]:
+            p = p * w + c
+        return p
+
+    g = np.random.default_rng(11)
+    x = np.concatenate([g.uniform(-1, 1, 150_000), g.uniform(0.49, 0.51, 20_000),
+                        1 - 10.0 ** g.uniform(-7, -1, 15_000), -1 + 10.0 ** g.uniform(-7, -1, 15_000),
+                        [-1.0, -0.5, 0.0, 0.5, 1.0]]).astype(np.float32).astype(np.float64)
+    a = np.abs(x)
+    small = a < 0.5
+    w_s = x * x
+    t_small = (pio2_hi - x) + (pio2_lo - (x * w_s) * poly(w_s))
+    w_b = (1.0 - a) * 0.5
+    s = np.sqrt(w_b)
+    q = s * w_b * poly(w_b) + s
+    t_big = np.where(x > 0, 2.0 * q, (pi_hi - 2.0 * q) + pi_lo)
+    got = np.where(small, t_small, t_big)
+    want = np.array([math.acos(v) for v in x])
+    ulp = np.spacing(np.maximum(np.abs(want), 1e-300))
+    err = np.abs(got - want) / ulp
+    assert err.max() < 4.0, float(err.max())
+    # rounded to float32 the two agree everywhere but within a few float64
+    # ulps of a float32 rounding boundary
+    assert np.mean(got.astype(np.float32) != want.astype(np.float32)) < 1e-4
