@@ -61,11 +61,11 @@ __device__ __forceinline__ void transmittance_body(const Consts& k, float bottom
     const bool active = texel_raw < n;
     const long texel = active ? texel_raw : n - 1;
     const int x = (int)(texel / k.th), y = (int)(texel % k.th);
-    float u = ((float)x + 0.5f) / (float)k.tw;
-    float v = ((float)y + 0.5f) / (float)k.th;
+    float u = cdiv((float)x + 0.5f, k.tw_f);
+    float v = cdiv((float)y + 0.5f, k.th_f);
     float alt, c, distance;
     transmittance_uv_to_ray(k, u, v, bottom, alt, c, distance);
-    float sine = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+    float sine = csqrt(fmaxf(1.0f - c * c, 0.0f));
     f3 origin = mk(0.0f, bottom + alt, 0.0f);
     f3 ray = mk(sine, c, 0.0f);
     Warp w = make_warp(-dot(origin, ray), 0.0f, distance);
@@ -77,11 +77,13 @@ __device__ __forceinline__ void transmittance_body(const Consts& k, float bottom
 #if ATMO_ROBUST
     f3 carry = mk(0.0f, 0.0f, 0.0f);  // Kahan compensation of the lane's chunk
 #endif
+    // each step's start is the previous step's end: the same boundary() value
+    float d0 = boundary(w, quot((float)s0, fsteps, rsteps));
     for (int s = s0; s < s1; ++s) {
-        float d0 = boundary(w, quot((float)s, fsteps, rsteps));
         float d1 = boundary(w, quot((float)(s + 1), fsteps, rsteps));
         float step = d1 - d0;
         float dist = (d0 + d1) * 0.5f;
+        d0 = d1;
 #if ATMO_ROBUST
         // |origin + ray d| - R cancels at planet scale (and the float32
         // origin radius is itself rounded): the float64 altitude on the ray
@@ -268,12 +270,12 @@ __device__ __forceinline__ void sky_view_body(const Consts& k, float bottom, flo
     const bool active = texel_raw < n;
     const long texel = active ? texel_raw : n - 1;
     const int x = (int)(texel / k.sh), y = (int)(texel % k.sh);
-    float sun_s = sqrtf(fmaxf(1.0f - sun_zc * sun_zc, 0.0f));
+    float sun_s = csqrt(fmaxf(1.0f - sun_zc * sun_zc, 0.0f));
     f3 sun = mk(sun_s, sun_zc, 0.0f);
     float safe_alt = fmaxf(cam_alt, 0.0f);
     f3 origin = mk(0.0f, bottom + safe_alt, 0.0f);
-    float az = ((float)x + 0.5f) / (float)k.sw * k.pi;
-    float view_v = ((float)y + 0.5f) / (float)k.sh;
+    float az = cdiv((float)x + 0.5f, k.sw_f) * k.pi;
+    float view_v = cdiv((float)y + 0.5f, k.sh_f);
 #if ATMO_ROBUST
     const double zenith_w = sky_v_to_zenith_w(view_v, safe_alt, bottom);
     float vc = (float)cos(zenith_w);
@@ -297,11 +299,12 @@ __device__ __forceinline__ void sky_view_body(const Consts& k, float bottom, flo
         const float fsteps = (float)steps, rsteps = 1.0f / fsteps;
         const int per = (steps + G - 1) / G;
         const int s0 = min(lane * per, steps), s1 = min(s0 + per, steps);
+        float d0 = boundary(w, quot((float)s0, fsteps, rsteps));
         for (int s = s0; s < s1; ++s) {
-            float d0 = boundary(w, quot((float)s, fsteps, rsteps));
             float d1 = boundary(w, quot((float)(s + 1), fsteps, rsteps));
             float dist = (d0 + d1) * 0.5f;
             float step = d1 - d0;
+            d0 = d1;
             f3 point = origin + ray * dist;
             float h = altitude_on_ray(safe_alt, vc, dist, bottom);
             f3 d = density(k, h);
