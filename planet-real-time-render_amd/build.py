@@ -40,6 +40,10 @@ HIPCC_FLAGS = [
     "-target-feature",
     "-Xclang",
     "-packed-fp32-ops",
+    # s_setprio around the early vector loads of a wave, so they issue ahead of
+    # other waves' ALU work: 3.6% on the composite, neutral elsewhere
+    "-mllvm",
+    "-amdgpu-set-wave-priority",
     "-Wall",
     "-Wno-unused-function",
 ]
