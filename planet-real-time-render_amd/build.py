@@ -5,8 +5,13 @@ Flags:
                       as written, matching the operation order of the
                       reference's formulas;
   -Xclang -target-feature -Xclang -packed-fp32-ops
-                      no packed-FP32 instructions (v_pk_*_f32).
-HIP's default IEEE division and square root are kept.
+                      no packed-FP32 instructions (v_pk_*_f32);
+  -mllvm -amdgpu-set-wave-priority
+                      the compiler raises a wave's priority around its early
+                      vector loads (scheduling only, no change to results).
+HIP's default IEEE division and square root stay the compiler's sequences;
+the kernels' short correctly rounded forms (atmo_device.h) give the same
+bits on their domains.
 """
 
 from __future__ import annotations
