@@ -10,7 +10,8 @@ The frame is timed twice: with the plain composite and with the learned
 surrogate composite.  Both are checked against the float64 CPU
 restatement; `value` is the faster form whose p99 relative error is within
 1 % (the plain form always is).  The 256-state batched LUT build is timed
-after the frames.
+after the frames, and the display post-process (bloom, ACES, sRGB) of the
+plain frame's HDR output after that.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -248,6 +249,57 @@ def bench_batch(pkg, scene, device, steps: int):
                     "half 50 km planet"}
 
 
+# Post-process algorithmic bytes per pixel: hdr read (12 B) and display
+# written (12 B); with bloom, plus bloom_a and bloom_b written once per
+# half-resolution texel (2 x 12 B per 4 pixels).
+POST_BYTES_PER_PIXEL = {"bloom": 12 + 12 + 6, "resolve_only": 12 + 12}
+
+
+def bench_postprocess(pkg, hdr, device, steps: int, with_oracle: bool):
+    """The display post-process (postprocess.py:176-207) of the plain
+    frame's HDR output: COMPOSITE (bloom + ACES/sRGB) and a raw view
+    (exposure-free clamp, no bloom), each timed with events around `steps`
+    back-to-back calls on the current stream."""
+    import torch
+
+    w, h = hdr.shape[:2]
+    config = pkg.PostprocessConfig()
+    post = pkg.PostProcessor(w, h, config, device)
+    display = torch.empty_like(hdr)
+    stream = torch.cuda.current_stream(device)
+    out = {"config": "PostprocessConfig() defaults: 4 blur pass pairs, threshold 1, knee 0.5, "
+                     "clamp 8, strength 0.08, ev 0"}
+    for name, bloom, tone_map in (("bloom", True, True), ("resolve_only", False, False)):
+        for _ in range(5):
+            post.process(hdr, display, 0.0, bloom=bloom, tone_map=tone_map)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize(device)
+        e0.record(stream)
+        for _ in range(steps):
+            post.process(hdr, display, 0.0, bloom=bloom, tone_map=tone_map)
+        e1.record(stream)
+        torch.cuda.synchronize(device)
+        us = e0.elapsed_time(e1) / steps * 1e3
+        nbytes = POST_BYTES_PER_PIXEL[name] * w * h
+        out[name] = {"us": round(us, 2), "algorithmic_bytes": nbytes,
+                     "hbm_gbs": round(nbytes / (us * 1e-6) / 1e9, 1),
+                     "hbm_frac": round(nbytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+    post.process(hdr, display, 0.0, bloom=True, tone_map=True)
+    torch.cuda.synchronize(device)
+    got = display.cpu().numpy()
+    if with_oracle:
+        from oracle.post_cpu import PostOracle
+
+        o = PostOracle(w, h, config)
+        c0 = time.perf_counter()
+        want = o.process(hdr.cpu().numpy(), 0.0, bloom=True, tone_map=True)
+        out["cpu_oracle_ms"] = round((time.perf_counter() - c0) * 1e3, 2)
+        out["display_values_differing_from_oracle"] = int(
+            (got.view(np.uint32) != want.view(np.uint32)).sum())
+    post.close()
+    return out
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -324,6 +376,14 @@ def main() -> int:
     if batch:
         log(f"batch: {batch['states']} states in {batch['ms_per_batch']:.3f} ms "
             f"({batch['states_per_s']:.0f} states/s)")
+
+    postprocess = bench_postprocess(pkg, torch.from_numpy(plain_hdr).to(device), device,
+                                    args.steps, not args.no_cpu_baseline)
+    log(f"post-process: bloom + tone map {postprocess['bloom']['us']:.2f} us "
+        f"({postprocess['bloom']['hbm_gbs']:.0f} GB/s algorithmic), resolve only "
+        f"{postprocess['resolve_only']['us']:.2f} us"
+        + (f"; {postprocess['display_values_differing_from_oracle']} display values differ "
+           f"from the oracle" if "cpu_oracle_ms" in postprocess else ""))
 
     n_pix = inputs.width * inputs.height
     surf_mask = inputs.surface_id >= 0
@@ -474,6 +534,7 @@ def main() -> int:
         "accuracy": accuracy,
         "parity": parity,
         "batch": batch,
+        "postprocess": postprocess,
     }
     frame.r.close()
     print(json.dumps(result), flush=True)

@@ -372,6 +372,63 @@ int32_t atmo_composite_surrogate(
     const float forward[3], float tangent_half_fov, float star_contrast_start,
     float star_contrast_end, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Display post-process: the reference's `PostProcessor`
+ * (src/planet_renderer/postprocess.py:74-207), the step after the composite
+ * (renderer.py:1729-1735).  Host policy stays in the caller, as
+ * `PostProcessor.process` keeps it (postprocess.py:176-207): the exposure
+ * multiplier 2**exposure_ev (1 without tone mapping) and the bloom switch
+ * (bloom requested, bloom_passes > 0, bloom_strength > 0).
+ * --------------------------------------------------------------------- */
+
+/* Mirrors every field of `PostprocessConfig` (postprocess.py:10-19), in the
+ * same order.  exposure_ev is only validated here: the caller derives the
+ * exposure multiplier. */
+typedef struct AtmoPostParams {
+    double exposure_ev;
+    double bloom_threshold;
+    double bloom_knee;
+    double bloom_clamp;
+    double bloom_strength;
+    int32_t bloom_passes;
+} AtmoPostParams;
+
+/* Handle-owned half-resolution fields of `PostProcessor.__init__`
+ * (postprocess.py:84-86), (max((W+1)/2, 1), max((H+1)/2, 1), 3) f32. */
+enum {
+    ATMO_POST_BUF_BLOOM_A = 0,
+    ATMO_POST_BUF_BLOOM_B = 1
+};
+
+typedef struct AtmoPost AtmoPost;
+
+/* Range checks of `PostprocessConfig.__post_init__` (postprocess.py:21-37),
+ * with the reference's ValueError messages. */
+int32_t atmo_post_validate_params(const AtmoPostParams* params, char* message,
+                                  int32_t message_capacity);
+/* Replaces `PostProcessor.__init__(width, height, config)`
+ * (postprocess.py:78-86). */
+int32_t atmo_post_create(const AtmoPostParams* params, int32_t width, int32_t height,
+                         AtmoPost** out);
+int32_t atmo_post_destroy(AtmoPost* post);
+const char* atmo_post_last_error(const AtmoPost* post);
+int32_t atmo_post_get_buffer(AtmoPost* post, int32_t which, void** device_ptr,
+                             int64_t shape[4]);
+/* Same contract as atmo_bind_buffer, for ATMO_POST_BUF_*. */
+int32_t atmo_post_bind_buffer(AtmoPost* post, int32_t which, void* device_ptr,
+                              int64_t bytes);
+/* Replaces the kernel sequence of `PostProcessor.process`
+ * (postprocess.py:195-207): `_extract(hdr, exposure_multiplier)` and
+ * bloom_passes x (`_blur_horizontal`, `_blur_vertical`) when use_bloom is
+ * non-zero, then `_resolve(hdr, display, exposure_multiplier,
+ * bloom_strength, use_bloom, tone_map)`.  hdr and display are (W,H,3) f32
+ * device pointers; hdr is only read.  Leaves the bloom fields as the
+ * reference does (bloom_a after the last vertical pass, bloom_b after the
+ * last horizontal pass). */
+int32_t atmo_post_process(AtmoPost* post, const float* hdr, float* display,
+                          float exposure_multiplier, int32_t use_bloom, int32_t tone_map,
+                          void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,13 +3,16 @@
 Drop-in for the reference's `planet_renderer.atmosphere` package
 (src/planet_renderer/atmosphere/__init__.py): `AtmosphereConfig`,
 `AtmosphereDiagnosticView`, `AtmosphereRenderer`, plus `LightingState`
-(src/planet_renderer/lighting.py).  Compute runs in libatmo.so (HIP,
+(src/planet_renderer/lighting.py), and the display post-process that
+follows the composite, `PostprocessConfig` / `PostProcessor`
+(src/planet_renderer/postprocess.py).  Compute runs in libatmo.so (HIP,
 gfx950); this package is its host side.
 """
 
 from .config import AtmosphereConfig
 from .diagnostics import AtmosphereDiagnosticView
 from .lighting import LightingState, StaticLightingProvider
+from .postprocess import PostprocessConfig, PostProcessor
 from .renderer import AtmosphereRenderer, DeviceField
 
 __all__ = [
@@ -18,5 +21,7 @@ __all__ = [
     "AtmosphereRenderer",
     "DeviceField",
     "LightingState",
+    "PostProcessor",
+    "PostprocessConfig",
     "StaticLightingProvider",
 ]

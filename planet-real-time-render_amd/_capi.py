@@ -67,7 +67,17 @@ EXPORTED_SYMBOLS = (
     "atmo_mlp_infer",
     "atmo_surrogate_labels",
     "atmo_composite_surrogate",
+    "atmo_post_validate_params",
+    "atmo_post_create",
+    "atmo_post_destroy",
+    "atmo_post_last_error",
+    "atmo_post_get_buffer",
+    "atmo_post_bind_buffer",
+    "atmo_post_process",
 )
+
+POST_BUF_BLOOM_A = 0
+POST_BUF_BLOOM_B = 1
 
 MLP_PARAMS = 9094
 
@@ -122,6 +132,19 @@ class AtmoParams(ctypes.Structure):
         ("aerial_steps_per_slice", ctypes.c_int32),
         ("aerial_horizon_raymarch_steps", ctypes.c_int32),
         ("aerial_terminator_substeps", ctypes.c_int32),
+    ]
+
+
+class AtmoPostParams(ctypes.Structure):
+    """struct AtmoPostParams (include/atmo.h), field for field."""
+
+    _fields_ = [
+        ("exposure_ev", ctypes.c_double),
+        ("bloom_threshold", ctypes.c_double),
+        ("bloom_knee", ctypes.c_double),
+        ("bloom_clamp", ctypes.c_double),
+        ("bloom_strength", ctypes.c_double),
+        ("bloom_passes", ctypes.c_int32),
     ]
 
 
@@ -203,8 +226,17 @@ def _signatures(lib: ctypes.CDLL) -> None:
         _P, _P, _P, _P, _P, _P, _P, _F, _F, _F, _PF3, _PF3, _PF3, _F, _PF3, _PF3, _PF3,
         _F, _F, _F, _P,
     ]
+    lib.atmo_post_validate_params.argtypes = [ctypes.POINTER(AtmoPostParams), ctypes.c_char_p, _I]
+    lib.atmo_post_create.argtypes = [ctypes.POINTER(AtmoPostParams), _I, _I, ctypes.POINTER(_P)]
+    lib.atmo_post_destroy.argtypes = [_P]
+    lib.atmo_post_last_error.argtypes = [_P]
+    lib.atmo_post_last_error.restype = ctypes.c_char_p
+    lib.atmo_post_get_buffer.argtypes = [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int64)]
+    lib.atmo_post_bind_buffer.argtypes = [_P, _I, _P, ctypes.c_int64]
+    lib.atmo_post_process.argtypes = [_P, _P, _P, _F, _I, _I, _P]
     for name in EXPORTED_SYMBOLS:
-        if name not in ("atmo_last_error", "atmo_mlp_last_error", "atmo_batch_last_error"):
+        if name not in ("atmo_last_error", "atmo_mlp_last_error", "atmo_batch_last_error",
+                        "atmo_post_last_error"):
             getattr(lib, name).restype = ctypes.c_int32
 
 
@@ -253,6 +285,7 @@ def check(status: int, handle=None) -> None:
 
 __all__ = [
     "AtmoBatchState",
+    "AtmoPostParams",
     "AtmoParams",
     "AtmoError",
     "EXPORTED_SYMBOLS",

@@ -9,11 +9,14 @@
 // reciprocal and quotient of the atmosphere kernels (atmo::csqrt, crcp,
 // cdiv) with the compiler's IEEE expansions over their stated domains, and
 // the kernels' float64 acos (atmo::acos_d) rounded to float32 with the
-// device library's acos rounded the same way.
+// device library's acos rounded the same way.  atmo_probe_srgb_pow
+// evaluates the post-process's sRGB power (post::srgb_pow) on an array, so
+// a test can compare it with the CPU oracle on every float of its domain.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "atmo_device.h"
+#include "post_device.h"
 
 namespace {
 
@@ -111,6 +114,13 @@ __global__ void __launch_bounds__(256) k_check_sincos(unsigned long long* counts
 
 }  // namespace
 
+// The display transform's sRGB power, one input per lane.
+__global__ void __launch_bounds__(256) k_srgb_pow(const float* __restrict__ in,
+                                                  float* __restrict__ out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = post::srgb_pow(in[i]);
+}
+
 extern "C" {
 
 // Runs the comparisons above on the null stream and copies the six
@@ -191,6 +201,16 @@ int32_t atmo_probe_exp_rate(int32_t blocks, int32_t iters, double* rate) {
     if (e != hipSuccess) return (int32_t)e;
     *rate = (double)blocks * 256.0 * (double)iters * kChains / (ms * 1.0e-3);
     return 0;
+}
+
+// out[i] = post::srgb_pow(in[i]) for i < n (device pointers, inputs in
+// (0.0031308, 1]) on `stream`.  Returns 0 on success, otherwise a
+// hipError_t value.
+int32_t atmo_probe_srgb_pow(const float* in, float* out, int64_t n, void* stream) {
+    if (!in || !out || n <= 0 || n > ((int64_t)1 << 36)) return (int32_t)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_srgb_pow, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, in, out, n);
+    return (int32_t)hipGetLastError();
 }
 
 }  // extern "C"
