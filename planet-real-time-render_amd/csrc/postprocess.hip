@@ -5,25 +5,31 @@
 // The reference runs 2 + 2P Taichi kernels per frame: `_extract` (2x2 box
 // downsample + bright pass into a half-resolution field), P passes of
 // `_blur_horizontal` + `_blur_vertical`, and `_resolve` (exposure,
-// bilinear bloom, ACES + sRGB or a [0, 1] clamp).  Here one kernel does
-// all of it per tile:
-//  - a 32x32 tile of half-resolution texels loads its halo of R = 2P + 1
-//    texels, extracts it from the HDR frame into LDS, and runs the P
-//    horizontal/vertical pass pairs in LDS (ping-pong, one barrier per
-//    pass), each pass over the shrinking region that the next one reads;
-//  - the tile's interior is written to bloom_a (last vertical pass) and
-//    bloom_b (last horizontal pass), the fields the reference leaves;
-//  - the tile's 64x64 full-resolution pixels are resolved from the blurred
-//    texels still in LDS (the bilinear footprint of those pixels stays
-//    within the tile plus one texel).
+// bilinear bloom, ACES + sRGB or a [0, 1] clamp).  Here three kernels:
+//  - k_post_extract: the bright pass, one half-resolution texel per lane,
+//    into a handle-owned field;
+//  - k_post_blur: all P pass pairs in LDS.  A 32x32 tile of that field
+//    loads its halo of 2P texels (every load issued before the first LDS
+//    store) and runs the passes ping-pong between two LDS copies, each
+//    pass over the shrinking region the next one reads.  A lane produces 4
+//    consecutive outputs along the pass axis from a sliding window of 8.
+//    Tiles whose halo lies inside the frame (82% at 1080p) run with
+//    compile-time regions and no clamping; the others clamp every window
+//    position as the reference clamps its taps.  The tile's interior goes
+//    to bloom_a (last vertical pass) and bloom_b (last horizontal pass),
+//    the fields the reference leaves;
+//  - k_post_resolve_bloom: four consecutive pixels per lane, 16-byte loads
+//    and stores, the bloom taps of all four pixels loaded before any
+//    arithmetic; without bloom every float of the frame is independent and
+//    k_post_resolve_flat streams the frame as float4s.
 // Every texel goes through the reference's operations in the reference's
 // order, so the result does not depend on the tiling: each pass reads the
-// same clamped neighbours the full-frame pass would.  Without bloom the
-// resolve runs alone, one pixel per lane.
+// same clamped neighbours the full-frame pass would.
 //
-// Tiles are dealt to the 8 XCDs in contiguous runs (block b runs on XCD
-// b mod 8), so neighbouring tiles, which read each other's halo, share an
-// L2.
+// Blur tiles and bloom-resolve chunks are dealt to the 8 XCDs in
+// contiguous runs (block b runs on XCD b mod 8), so neighbouring tiles,
+// which read each other's halo, and neighbouring pixels, which read the
+// same bloom texels, share an L2.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -41,6 +47,7 @@ namespace {
 constexpr uint32_t kPostMagic = 0x504f5354u;  // "POST"
 constexpr int kTile = 32;                     // half-resolution texels per tile side
 constexpr int kThreads = 256;
+constexpr int kBlurThreads = 512;
 constexpr int kXcds = 8;
 
 struct PostArgs {
@@ -48,8 +55,10 @@ struct PostArgs {
     float* display;
     float* bloom_a;
     float* bloom_b;
+    float* extracted;
     int32_t width, height, half_width, half_height;
     float width_f, height_f, half_width_f, half_height_f;
+    double inv_height;
     float exposure_multiplier, bloom_strength;
     post::BrightPass bright;
     int32_t tone_map;
@@ -60,13 +69,140 @@ __device__ __forceinline__ float output_channel(float value, int32_t tone_map) {
     return tone_map ? post::display_channel(value) : fminf(fmaxf(value, 0.0f), 1.0f);
 }
 
-// _extract + P x (_blur_horizontal, _blur_vertical) + _resolve, one tile per
-// block (postprocess.py:110-174).
+// _extract (postprocess.py:110-119): one half-resolution texel per lane,
+// blocks of 4 columns x 64 rows, lanes along y (the contiguous axis).
+__global__ __launch_bounds__(256) void k_post_extract(PostArgs a) {
+    const int gy = (int)blockIdx.x * 64 + ((int)threadIdx.x & 63);
+    const int gx = (int)blockIdx.y * 4 + ((int)threadIdx.x >> 6);
+    if (gy >= a.half_height || gx >= a.half_width) return;
+    float r = 0.0f, g = 0.0f, b = 0.0f;
+#pragma unroll
+    for (int ox = 0; ox < 2; ++ox) {
+        const int sx = min(gx * 2 + ox, a.width - 1);
+#pragma unroll
+        for (int oy = 0; oy < 2; ++oy) {
+            const int sy = min(gy * 2 + oy, a.height - 1);
+            const float* p = a.hdr + ((size_t)sx * a.height + sy) * 3;
+            r += p[0];
+            g += p[1];
+            b += p[2];
+        }
+    }
+    const float s = 0.25f * a.exposure_multiplier;
+    r *= s;
+    g *= s;
+    b *= s;
+    post::soft_threshold(a.bright, r, g, b);
+    float* o = a.extracted + ((size_t)gx * a.half_height + gy) * 3;
+    o[0] = r;
+    o[1] = g;
+    o[2] = b;
+}
+
+// One blur pass over the local region [lo0, hi0) x [lo1, hi1) of the tile,
+// src -> dst, along axis 0 (x, _blur_horizontal) or axis 1 (y,
+// _blur_vertical) (postprocess.py:121-139).  A lane produces S consecutive
+// outputs along the pass axis from a window of S + 4 texels; window
+// positions are clamped to the frame as the reference clamps each tap, then
+// into the tile so no access leaves LDS.
+template <int AXIS, int L, int LP, int N, int S>
+__device__ __forceinline__ void blur_pass(const float* __restrict__ src, float* __restrict__ dst,
+                                          int lo0, int hi0, int lo1, int hi1, int g0, int extent) {
+    // axis 0: segments along x, lanes along y; axis 1: segments along y, lanes along x
+    const int lane_lo = AXIS == 0 ? lo1 : lo0, lane_n = AXIS == 0 ? hi1 - lo1 : hi0 - lo0;
+    const int seg_lo = AXIS == 0 ? lo0 : lo1, seg_hi = AXIS == 0 ? hi0 : hi1;
+    const int nseg = (seg_hi - seg_lo + S - 1) / S;
+    if (lane_n <= 0) return;
+    const float inv_lane_n = 1.0f / (float)lane_n;  // item / lane_n below 2^12: exact
+    for (int item = (int)threadIdx.x; item < lane_n * nseg; item += kBlurThreads) {
+        const int seg = (int)(((float)item + 0.5f) * inv_lane_n);
+        const int across = lane_lo + item - seg * lane_n;
+        const int start = seg_lo + seg * S;
+        const int count = min(S, seg_hi - start);
+        int idx[S + 4];
+#pragma unroll
+        for (int k = 0; k < S + 4; ++k) {
+            const int gpos = min(max(g0 + start - 2 + k, 0), extent - 1);
+            const int l = min(max(gpos - g0, 0), L - 1);
+            idx[k] = AXIS == 0 ? l * LP + across : across * LP + l;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float w[S + 4];
+#pragma unroll
+            for (int k = 0; k < S + 4; ++k) w[k] = src[c * N + idx[k]];
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+                if (j < count) {
+                    float color = w[j + 2] * post::kW0;
+                    color += w[j + 1] * post::kW1;
+                    color += w[j + 3] * post::kW1;
+                    color += w[j] * post::kW2;
+                    color += w[j + 4] * post::kW2;
+                    dst[c * N + idx[j + 2]] = color;
+                }
+            }
+        }
+    }
+}
+
+// The same pass on a tile whose halo lies inside the frame: no tap is
+// clamped and every region is known at compile time (pass k: x in
+// [2k, L-2k); the horizontal pass keeps y in [2k-2, L-2k+2), the vertical
+// y in [2k, L-2k)), all multiples of S = 4 long, so a lane's window reads
+// are fixed offsets from one LDS address.
+template <int AXIS, int L, int LP, int N, int S, int ACROSS_LO, int ACROSS_N, int SEG_LO, int NSEG>
+__device__ __forceinline__ void blur_interior(const float* __restrict__ src,
+                                              float* __restrict__ dst) {
+    constexpr int STEP = AXIS == 0 ? LP : 1, ASTEP = AXIS == 0 ? 1 : LP;
+    for (int item = (int)threadIdx.x; item < NSEG * ACROSS_N; item += kBlurThreads) {
+        const int seg = item / ACROSS_N;
+        const int across = ACROSS_LO + item - seg * ACROSS_N;
+        const int start = SEG_LO + seg * S;
+        // element (position p along the pass axis, across) at p * STEP + across * ASTEP
+        const int base = (start - 2) * STEP + across * ASTEP;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float* s = src + c * N + base;
+            float* d = dst + c * N + base;
+            float w[S + 4];
+#pragma unroll
+            for (int k = 0; k < S + 4; ++k) w[k] = s[k * STEP];
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+                float color = w[j + 2] * post::kW0;
+                color += w[j + 1] * post::kW1;
+                color += w[j + 3] * post::kW1;
+                color += w[j] * post::kW2;
+                color += w[j + 4] * post::kW2;
+                d[(j + 2) * STEP] = color;
+            }
+        }
+    }
+}
+
+template <int L, int LP, int N, int K, int P>
+__device__ __forceinline__ void interior_passes(float* A, float* B) {
+    if constexpr (K <= P) {
+        constexpr int S = 4, NSEG = (L - 4 * K) / 4;
+        blur_interior<0, L, LP, N, S, 2 * K - 2, L - 4 * K + 4, 2 * K, NSEG>(A, B);
+        __syncthreads();
+        blur_interior<1, L, LP, N, S, 2 * K, L - 4 * K, 2 * K, NSEG>(B, A);
+        __syncthreads();
+        interior_passes<L, LP, N, K + 1, P>(A, B);
+    }
+}
+
+// P x (_blur_horizontal, _blur_vertical) on one 32x32 tile of the
+// extracted field with its halo of 2P texels, in LDS; writes the tile's
+// bloom_a (last vertical pass) and bloom_b (last horizontal pass).
 template <int P>
-__global__ __launch_bounds__(kThreads) void k_post_bloom(PostArgs a) {
-    constexpr int R = 2 * P + 1;       // halo: P pass pairs of 2 texels, + 1 for the bilinear
-    constexpr int L = kTile + 2 * R;   // local side
-    constexpr int N = L * L;
+__global__ __launch_bounds__(kBlurThreads) void k_post_blur(PostArgs a) {
+    constexpr int R = 2 * P;
+    constexpr int L = kTile + 2 * R;
+    constexpr int LP = L | 1;             // odd pitch: column walks hit distinct banks
+    constexpr int N = L * LP;
+    constexpr int S = 4;                  // outputs per lane along the pass axis
     __shared__ float lds[2 * 3 * N];
     float* A = lds;
     float* B = lds + 3 * N;
@@ -74,106 +210,59 @@ __global__ __launch_bounds__(kThreads) void k_post_bloom(PostArgs a) {
     const int tile = (int)(blockIdx.x % kXcds) * a.tiles_per_xcd + (int)(blockIdx.x / kXcds);
     if (tile >= a.tiles_x * a.tiles_y) return;  // whole block, before any barrier
     const int X0 = (tile / a.tiles_y) * kTile, Y0 = (tile % a.tiles_y) * kTile;
-    const int gx0 = X0 - R, gy0 = Y0 - R;  // global texel of local (0, 0)
+    const int gx0 = X0 - R, gy0 = Y0 - R;
     const int hw = a.half_width, hh = a.half_height;
 
-    // _extract (postprocess.py:110-119) over the whole local region; texels
-    // outside the frame are zero and never read.
-    for (int i = threadIdx.x; i < N; i += kThreads) {
+    // every global load of the halo tile is issued before the first LDS store
+    constexpr int NL = (L * L + kBlurThreads - 1) / kBlurThreads;
+    float v[NL][3];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int i = (int)threadIdx.x + k * kBlurThreads;
         const int lx = i / L, ly = i - lx * L;
         const int gx = gx0 + lx, gy = gy0 + ly;
-        float r = 0.0f, g = 0.0f, b = 0.0f;
-        if (gx >= 0 && gx < hw && gy >= 0 && gy < hh) {
-#pragma unroll
-            for (int ox = 0; ox < 2; ++ox) {
-                const int sx = min(gx * 2 + ox, a.width - 1);
-#pragma unroll
-                for (int oy = 0; oy < 2; ++oy) {
-                    const int sy = min(gy * 2 + oy, a.height - 1);
-                    const float* p = a.hdr + ((size_t)sx * a.height + sy) * 3;
-                    r += p[0];
-                    g += p[1];
-                    b += p[2];
-                }
-            }
-            const float s = 0.25f * a.exposure_multiplier;
-            r *= s;
-            g *= s;
-            b *= s;
-            post::soft_threshold(a.bright, r, g, b);
+        v[k][0] = v[k][1] = v[k][2] = 0.0f;
+        if (i < L * L && gx >= 0 && gx < hw && gy >= 0 && gy < hh) {
+            const float* p = a.extracted + ((size_t)gx * hh + gy) * 3;
+            v[k][0] = p[0];
+            v[k][1] = p[1];
+            v[k][2] = p[2];
         }
-        A[i] = r;
-        A[N + i] = g;
-        A[2 * N + i] = b;
+    }
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int i = (int)threadIdx.x + k * kBlurThreads;
+        if (i < L * L) {
+            const int lx = i / L, ly = i - lx * L;
+            const int li = lx * LP + ly;
+            A[li] = v[k][0];
+            A[N + li] = v[k][1];
+            A[2 * N + li] = v[k][2];
+        }
     }
     __syncthreads();
-
-    for (int pass = 1; pass <= P; ++pass) {
-        // _blur_horizontal (postprocess.py:121-129): A -> B on the texels
-        // the following vertical pass reads.
-        {
+    if (gx0 >= 0 && gx0 + L <= hw && gy0 >= 0 && gy0 + L <= hh) {
+        interior_passes<L, LP, N, 1, P>(A, B);
+    } else {
+#pragma unroll 1
+        for (int pass = 1; pass <= P; ++pass) {
             const int xlo = max(gx0 + 2 * pass, 0) - gx0;
             const int xhi = min(X0 + kTile + R - 2 * pass, hw) - gx0;
-            const int ylo = max(gy0 + 2 * (pass - 1), 0) - gy0;
-            const int yhi = min(Y0 + kTile + R - 2 * (pass - 1), hh) - gy0;
-            for (int i = threadIdx.x; i < N; i += kThreads) {
-                const int lx = i / L, ly = i - lx * L;
-                if (lx < xlo || lx >= xhi || ly < ylo || ly >= yhi) continue;
-                const int gx = gx0 + lx;
-                const int m1 = (max(gx - 1, 0) - gx0) * L + ly;
-                const int p1 = (min(gx + 1, hw - 1) - gx0) * L + ly;
-                const int m2 = (max(gx - 2, 0) - gx0) * L + ly;
-                const int p2 = (min(gx + 2, hw - 1) - gx0) * L + ly;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const float* s = A + c * N;
-                    float color = s[i] * post::kW0;
-                    color += s[m1] * post::kW1;
-                    color += s[p1] * post::kW1;
-                    color += s[m2] * post::kW2;
-                    color += s[p2] * post::kW2;
-                    B[c * N + i] = color;
-                }
-            }
-        }
-        __syncthreads();
-        // _blur_vertical (postprocess.py:131-139): B -> A.
-        {
-            const int xlo = max(gx0 + 2 * pass, 0) - gx0;
-            const int xhi = min(X0 + kTile + R - 2 * pass, hw) - gx0;
+            const int ylo_h = max(gy0 + 2 * (pass - 1), 0) - gy0;
+            const int yhi_h = min(Y0 + kTile + R - 2 * (pass - 1), hh) - gy0;
+            blur_pass<0, L, LP, N, S>(A, B, xlo, xhi, ylo_h, yhi_h, gx0, hw);
+            __syncthreads();
             const int ylo = max(gy0 + 2 * pass, 0) - gy0;
             const int yhi = min(Y0 + kTile + R - 2 * pass, hh) - gy0;
-            for (int i = threadIdx.x; i < N; i += kThreads) {
-                const int lx = i / L, ly = i - lx * L;
-                if (lx < xlo || lx >= xhi || ly < ylo || ly >= yhi) continue;
-                const int gy = gy0 + ly;
-                const int row = lx * L;
-                const int m1 = row + max(gy - 1, 0) - gy0;
-                const int p1 = row + min(gy + 1, hh - 1) - gy0;
-                const int m2 = row + max(gy - 2, 0) - gy0;
-                const int p2 = row + min(gy + 2, hh - 1) - gy0;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const float* s = B + c * N;
-                    float color = s[i] * post::kW0;
-                    color += s[m1] * post::kW1;
-                    color += s[p1] * post::kW1;
-                    color += s[m2] * post::kW2;
-                    color += s[p2] * post::kW2;
-                    A[c * N + i] = color;
-                }
-            }
+            blur_pass<1, L, LP, N, S>(B, A, xlo, xhi, ylo, yhi, gy0, hh);
+            __syncthreads();
         }
-        __syncthreads();
     }
-
-    // The fields the reference leaves behind: bloom_a after the last
-    // vertical pass, bloom_b after the last horizontal pass.
-    for (int i = threadIdx.x; i < kTile * kTile; i += kThreads) {
+    for (int i = threadIdx.x; i < kTile * kTile; i += kBlurThreads) {
         const int tx = i / kTile, ty = i - tx * kTile;
         const int gx = X0 + tx, gy = Y0 + ty;
         if (gx >= hw || gy >= hh) continue;
-        const int li = (tx + R) * L + (ty + R);
+        const int li = (tx + R) * LP + (ty + R);
         const size_t o = ((size_t)gx * hh + gy) * 3;
         a.bloom_a[o] = A[li];
         a.bloom_a[o + 1] = A[N + li];
@@ -182,62 +271,133 @@ __global__ __launch_bounds__(kThreads) void k_post_bloom(PostArgs a) {
         a.bloom_b[o + 1] = B[N + li];
         a.bloom_b[o + 2] = B[2 * N + li];
     }
+}
 
-    // _resolve (postprocess.py:155-174) with _sample_bloom (141-153) for the
-    // tile's full-resolution pixels.  Each lane keeps one y (and so one v)
-    // and steps x.
-    constexpr int F = 2 * kTile;
-    const int fy = (int)threadIdx.x % F;
-    const int y = 2 * Y0 + fy;
-    if (y >= a.height) return;  // no barrier follows
-    const float v = ((float)y + 0.5f) / a.height_f;
-    const float py = v * a.half_height_f - 0.5f;
-    const float fpy = floorf(py);
-    const int y0t = min(max((int)fpy, 0), hh - 1);
-    const int y1t = min(y0t + 1, hh - 1);
-    const float wy = fminf(fmaxf(py - fpy, 0.0f), 1.0f);
-    // local rows, clamped into the tile so no access leaves LDS
-    const int ly0 = min(max(y0t - gy0, 0), L - 1), ly1 = min(max(y1t - gy0, 0), L - 1);
-    for (int fx = (int)threadIdx.x / F; fx < F; fx += kThreads / F) {
-        const int x = 2 * X0 + fx;
-        if (x >= a.width) break;
-        const float u = ((float)x + 0.5f) / a.width_f;
-        const float px = u * a.half_width_f - 0.5f;
-        const float fpx = floorf(px);
-        const int x0t = min(max((int)fpx, 0), hw - 1);
-        const int x1t = min(x0t + 1, hw - 1);
-        const float wx = fminf(fmaxf(px - fpx, 0.0f), 1.0f);
-        const int lx0 = min(max(x0t - gx0, 0), L - 1) * L, lx1 = min(max(x1t - gx0, 0), L - 1) * L;
-        const size_t o = ((size_t)x * a.height + y) * 3;
-        const float* h = a.hdr + o;
-        float* d = a.display + o;
+// _sample_bloom's texel pair and weight along one axis (postprocess.py:141-150).
+struct Lerp {
+    int i0, i1;
+    float w;
+};
+__device__ __forceinline__ Lerp bloom_axis(int pixel, float size_f, float half_size_f, int half_size) {
+    const float t = ((float)pixel + 0.5f) / size_f;
+    const float p = t * half_size_f - 0.5f;
+    const float fp = floorf(p);
+    Lerp l;
+    l.i0 = min(max((int)fp, 0), half_size - 1);
+    l.i1 = min(l.i0 + 1, half_size - 1);
+    l.w = fminf(fmaxf(p - fp, 0.0f), 1.0f);
+    return l;
+}
+
+// _resolve with bloom (postprocess.py:155-174): four consecutive pixels per
+// lane (16-byte loads and stores when VEC), bloom taps from bloom_a.  Lanes
+// past the frame's last pixel compute on clamped taps and store nothing.
+template <bool VEC>
+__global__ __launch_bounds__(kThreads) void k_post_resolve_bloom(PostArgs a) {
+    const size_t n = (size_t)a.width * a.height;
+    const size_t chunk = (size_t)(blockIdx.x % kXcds) * a.tiles_per_xcd + blockIdx.x / kXcds;
+    const size_t i0 = (chunk * kThreads + threadIdx.x) * 4;
+    if (i0 >= n) return;
+    const int npx = (int)min((size_t)4, n - i0);
+    float v[12];
+    if (VEC && npx == 4) {
+        const float4* h4 = reinterpret_cast<const float4*>(a.hdr + i0 * 3);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float* s = A + c * N;
-            const float low = s[lx0 + ly0] * (1.0f - wx) + s[lx1 + ly0] * wx;
-            const float high = s[lx0 + ly1] * (1.0f - wx) + s[lx1 + ly1] * wx;
-            const float sample = low * (1.0f - wy) + high * wy;
-            float value = h[c] * a.exposure_multiplier;
-            value += sample * a.bloom_strength;
-            d[c] = output_channel(value, a.tone_map);
+        for (int k = 0; k < 3; ++k) {
+            const float4 q = h4[k];
+            v[4 * k] = q.x;
+            v[4 * k + 1] = q.y;
+            v[4 * k + 2] = q.z;
+            v[4 * k + 3] = q.w;
         }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) v[k] = k < npx * 3 ? a.hdr[i0 * 3 + k] : 0.0f;
+    }
+    // pixel i0 is (x, y) with i0 = x * H + y; exact for any frame below
+    // 2^51 pixels
+    const int x = (int)(((double)i0 + 0.5) * a.inv_height);
+    const int y = (int)(i0 - (size_t)x * a.height);
+    {
+        // bilinear set-up and every tap load of the four pixels first, then
+        // the arithmetic (the tone map's rare slow path is a call, which
+        // loads do not cross)
+        float tap[4][4][3];
+        float wx[4], wy[4];
+        int px_x = x, px_y = y;
+        Lerp lx = bloom_axis(px_x, a.width_f, a.half_width_f, a.half_width);
+        const int hh = a.half_height;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const Lerp ly = bloom_axis(px_y, a.height_f, a.half_height_f, a.half_height);
+            wx[p] = lx.w;
+            wy[p] = ly.w;
+            const float* b00 = a.bloom_a + ((size_t)lx.i0 * hh + ly.i0) * 3;
+            const float* b10 = a.bloom_a + ((size_t)lx.i1 * hh + ly.i0) * 3;
+            const float* b01 = a.bloom_a + ((size_t)lx.i0 * hh + ly.i1) * 3;
+            const float* b11 = a.bloom_a + ((size_t)lx.i1 * hh + ly.i1) * 3;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                tap[p][0][c] = b00[c];
+                tap[p][1][c] = b10[c];
+                tap[p][2][c] = b01[c];
+                tap[p][3][c] = b11[c];
+            }
+            if (++px_y == a.height && p < 3) {
+                px_y = 0;
+                ++px_x;
+                lx = bloom_axis(px_x, a.width_f, a.half_width_f, a.half_width);
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float low = tap[p][0][c] * (1.0f - wx[p]) + tap[p][1][c] * wx[p];
+                const float high = tap[p][2][c] * (1.0f - wx[p]) + tap[p][3][c] * wx[p];
+                const float sample = low * (1.0f - wy[p]) + high * wy[p];
+                float value = v[3 * p + c] * a.exposure_multiplier;
+                value += sample * a.bloom_strength;
+                v[3 * p + c] = output_channel(value, a.tone_map);
+            }
+        }
+    }
+    if (VEC && npx == 4) {
+        float4* d4 = reinterpret_cast<float4*>(a.display + i0 * 3);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            d4[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k)
+            if (k < npx * 3) a.display[i0 * 3 + k] = v[k];
     }
 }
 
-// _resolve without bloom (postprocess.py:155-174, use_bloom == 0).
-__global__ __launch_bounds__(kThreads) void k_post_resolve(PostArgs a) {
-    const size_t n = (size_t)a.width * a.height;
-    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
-    const float* h = a.hdr + i * 3;
-    float* d = a.display + i * 3;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) d[c] = output_channel(h[c] * a.exposure_multiplier, a.tone_map);
+// _resolve without bloom: every float of the frame independently
+// (exposure, then ACES + sRGB or the [0, 1] clamp), four per lane.
+template <bool VEC>
+__global__ __launch_bounds__(kThreads) void k_post_resolve_flat(PostArgs a) {
+    const size_t n = (size_t)a.width * a.height * 3;
+    const size_t i0 = ((size_t)blockIdx.x * kThreads + threadIdx.x) * 4;
+    if (i0 >= n) return;
+    if (VEC && i0 + 4 <= n) {
+        const float4 q = *reinterpret_cast<const float4*>(a.hdr + i0);
+        float4 o;
+        o.x = output_channel(q.x * a.exposure_multiplier, a.tone_map);
+        o.y = output_channel(q.y * a.exposure_multiplier, a.tone_map);
+        o.z = output_channel(q.z * a.exposure_multiplier, a.tone_map);
+        o.w = output_channel(q.w * a.exposure_multiplier, a.tone_map);
+        *reinterpret_cast<float4*>(a.display + i0) = o;
+    } else {
+        for (size_t i = i0; i < i0 + 4 && i < n; ++i)
+            a.display[i] = output_channel(a.hdr[i] * a.exposure_multiplier, a.tone_map);
+    }
 }
 
 template <int P>
-hipError_t launch_bloom(const PostArgs& a, hipStream_t stream) {
-    hipLaunchKernelGGL(k_post_bloom<P>, dim3(kXcds * a.tiles_per_xcd), dim3(kThreads), 0, stream, a);
+hipError_t launch_blur(const PostArgs& a, hipStream_t stream) {
+    hipLaunchKernelGGL(k_post_blur<P>, dim3(kXcds * a.tiles_per_xcd), dim3(kBlurThreads), 0, stream, a);
     return hipGetLastError();
 }
 
@@ -251,6 +411,7 @@ struct AtmoPost {
     float bloom_strength;
     float* bloom[2];
     bool borrowed[2];
+    float* extracted;  // _extract's output, read by the blur tiles
     char error[256];
 };
 
@@ -337,16 +498,17 @@ int32_t atmo_post_create(const AtmoPostParams* params, int32_t width, int32_t he
     h->bright.knee_denominator = (float)fmax(4.0 * knee, 1.0e-6);
     h->bright.clamp = (float)params->bloom_clamp;
     h->bloom_strength = (float)params->bloom_strength;
-    for (int i = 0; i < 2; ++i) {
-        if (hipMalloc(&h->bloom[i], bloom_floats(h) * sizeof(float)) != hipSuccess ||
-            hipMemset(h->bloom[i], 0, bloom_floats(h) * sizeof(float)) != hipSuccess) {
-            post_error(nullptr, "atmo_post_create: device allocation failed");
-            for (int j = 0; j <= i; ++j)
-                if (h->bloom[j]) (void)hipFree(h->bloom[j]);
-            h->magic = 0;
-            delete h;
-            return ATMO_ENOMEM;
-        }
+    const size_t bytes = bloom_floats(h) * sizeof(float);
+    bool ok = true;
+    for (float** p : {&h->bloom[0], &h->bloom[1], &h->extracted})
+        ok = ok && hipMalloc(p, bytes) == hipSuccess && hipMemset(*p, 0, bytes) == hipSuccess;
+    if (!ok) {
+        post_error(nullptr, "atmo_post_create: device allocation failed");
+        for (float* p : {h->bloom[0], h->bloom[1], h->extracted})
+            if (p) (void)hipFree(p);
+        h->magic = 0;
+        delete h;
+        return ATMO_ENOMEM;
     }
     *out = h;
     return ATMO_OK;
@@ -357,6 +519,7 @@ int32_t atmo_post_destroy(AtmoPost* h) {
     (void)hipDeviceSynchronize();
     for (int i = 0; i < 2; ++i)
         if (h->bloom[i] && !h->borrowed[i]) (void)hipFree(h->bloom[i]);
+    if (h->extracted) (void)hipFree(h->extracted);
     h->magic = 0;
     delete h;
     return ATMO_OK;
@@ -431,26 +594,45 @@ int32_t atmo_post_process(AtmoPost* h, const float* hdr, float* display,
     a.bloom_strength = h->bloom_strength;
     a.bright = h->bright;
     a.tone_map = tone_map != 0;
+    a.extracted = h->extracted;
+    a.inv_height = 1.0 / (double)h->height;
     hipStream_t s = (hipStream_t)stream;
-    hipError_t e;
+    hipError_t e = hipSuccess;
     if (use_bloom) {
+        hipLaunchKernelGGL(k_post_extract, dim3((h->half_height + 63) / 64, (h->half_width + 3) / 4),
+                           dim3(256), 0, s, a);
         a.tiles_x = (h->half_width + kTile - 1) / kTile;
         a.tiles_y = (h->half_height + kTile - 1) / kTile;
         a.tiles_per_xcd = (a.tiles_x * a.tiles_y + kXcds - 1) / kXcds;
         switch (h->params.bloom_passes) {
-            case 1: e = launch_bloom<1>(a, s); break;
-            case 2: e = launch_bloom<2>(a, s); break;
-            case 3: e = launch_bloom<3>(a, s); break;
-            case 4: e = launch_bloom<4>(a, s); break;
-            case 5: e = launch_bloom<5>(a, s); break;
-            case 6: e = launch_bloom<6>(a, s); break;
-            case 7: e = launch_bloom<7>(a, s); break;
-            default: e = launch_bloom<8>(a, s); break;
+            case 1: e = launch_blur<1>(a, s); break;
+            case 2: e = launch_blur<2>(a, s); break;
+            case 3: e = launch_blur<3>(a, s); break;
+            case 4: e = launch_blur<4>(a, s); break;
+            case 5: e = launch_blur<5>(a, s); break;
+            case 6: e = launch_blur<6>(a, s); break;
+            case 7: e = launch_blur<7>(a, s); break;
+            default: e = launch_blur<8>(a, s); break;
         }
-    } else {
+    }
+    if (e == hipSuccess) {
         const size_t n = (size_t)h->width * h->height;
-        hipLaunchKernelGGL(k_post_resolve, dim3((unsigned)((n + kThreads - 1) / kThreads)),
-                           dim3(kThreads), 0, s, a);
+        const bool vec = (((uintptr_t)hdr | (uintptr_t)display) & 15) == 0;
+        if (use_bloom) {
+            const size_t chunks = (n + 4 * kThreads - 1) / (4 * kThreads);
+            a.tiles_per_xcd = (int32_t)((chunks + kXcds - 1) / kXcds);
+            const dim3 grid((unsigned)(kXcds * a.tiles_per_xcd));
+            if (vec)
+                hipLaunchKernelGGL(k_post_resolve_bloom<true>, grid, dim3(kThreads), 0, s, a);
+            else
+                hipLaunchKernelGGL(k_post_resolve_bloom<false>, grid, dim3(kThreads), 0, s, a);
+        } else {
+            const dim3 grid((unsigned)((3 * n + 4 * kThreads - 1) / (4 * kThreads)));
+            if (vec)
+                hipLaunchKernelGGL((k_post_resolve_flat<true>), grid, dim3(kThreads), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_post_resolve_flat<false>), grid, dim3(kThreads), 0, s, a);
+        }
         e = hipGetLastError();
     }
     if (e != hipSuccess) {
