@@ -11,6 +11,7 @@ gfx950); this package is its host side.
 
 from .config import AtmosphereConfig
 from .diagnostics import AtmosphereDiagnosticView
+from .frame_io import OCEAN_SURFACE_ID, check_gbuffer, load_hdr, save_hdr
 from .lighting import LightingState, StaticLightingProvider
 from .postprocess import PostprocessConfig, PostProcessor
 from .renderer import AtmosphereRenderer, DeviceField
@@ -21,7 +22,11 @@ __all__ = [
     "AtmosphereRenderer",
     "DeviceField",
     "LightingState",
+    "OCEAN_SURFACE_ID",
     "PostProcessor",
     "PostprocessConfig",
     "StaticLightingProvider",
+    "check_gbuffer",
+    "load_hdr",
+    "save_hdr",
 ]

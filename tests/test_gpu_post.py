@@ -99,9 +99,10 @@ def test_post_process_bit_exact(pkg, device, size, changes, ev, bloom, tone_map)
     assert_bit_exact("bloom_b", got[2], want[2])
 
 
-def test_post_process_of_a_composite_frame(pkg, scene, device):
+def test_post_process_of_a_composite_frame(pkg, scene, device, tmp_path):
     """The frame the composite writes (1920x1080, COMPOSITE view), then the
-    post-process the reference runs on it (renderer.py:1728-1735)."""
+    post-process the reference runs on it (renderer.py:1728-1735), and the
+    linear-HDR file the reference saves from it (cli.py:143-145)."""
     import torch
 
     from test_gpu_parity import gpu_frame
@@ -117,7 +118,9 @@ def test_post_process_of_a_composite_frame(pkg, scene, device):
     for name, g, w in zip(("display", "bloom_a", "bloom_b"), got, want):
         assert_bit_exact(name, g, w)
     assert float(want[1].max()) > 0.0  # the frame has a bright pass to blur
-    torch.cuda.synchronize(device)
+    pkg.save_hdr(torch.from_numpy(hdr).to(device), tmp_path / "frame.npy")
+    np.save(tmp_path / "reference.npy", hdr)
+    assert (tmp_path / "frame.npy").read_bytes() == (tmp_path / "reference.npy").read_bytes()
 
 
 def test_srgb_power_on_every_float_of_its_domain(device):
