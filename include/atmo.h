@@ -429,6 +429,65 @@ int32_t atmo_post_process(AtmoPost* post, const float* hdr, float* display,
                           float exposure_multiplier, int32_t use_bloom, int32_t tone_map,
                           void* stream);
 
+/* ------------------------------------------------------------------------
+ * Surface shading: the reference's `PlanetRenderer._shade_surface`
+ * (src/planet_renderer/renderer.py:1277-1575), the consumer of
+ * prepare_frame's surface_sun_transmittance and surface_sky_radiance
+ * (renderer.py:1682-1702).  Lambert terrain lighting, and for ocean pixels
+ * (surface_id == 2147000000, ocean.py:17) with the advanced ocean enabled:
+ * GGX sun specular, Fresnel sky reflection, a refracted screen-space seabed
+ * lookup into the pre-ocean terrain snapshot, and water absorption and
+ * scattering.
+ * --------------------------------------------------------------------- */
+
+/* The renderer's fields the kernel reads and writes (renderer.py:235-252),
+ * all (W,H[,k]) device arrays in the Taichi field layout: f3 = 3 floats. */
+typedef struct AtmoShadeFields {
+    const int32_t* surface_id;          /* gbuffer_surface_id (W,H) */
+    const float* albedo;                /* gbuffer_albedo f3 */
+    const float* normal;                /* gbuffer_normal f3 (global) */
+    const float* position;              /* gbuffer_position f3 (view) */
+    const float* ocean_slope_variance;  /* gbuffer_ocean_slope_variance (W,H) */
+    const float* water_depth_m;         /* gbuffer_water_depth_m (W,H) */
+    const float* seabed_albedo;         /* gbuffer_seabed_albedo f3 */
+    const float* seabed_normal;         /* gbuffer_seabed_normal f3 */
+    const int32_t* terrain_surface_id;  /* terrain snapshot (W,H) */
+    const float* terrain_position;      /* f3 */
+    const float* terrain_albedo;        /* f3 */
+    const float* terrain_normal;        /* f3 */
+    const float* sun_transmittance;     /* prepare_frame output f3 */
+    const float* sky_radiance;          /* prepare_frame output f3 */
+    float* surface_hdr;                 /* output f3 */
+} AtmoShadeFields;
+
+/* The kernel's scalar arguments (renderer.py:1279-1300), float32 as the
+ * `ti.f32` parameters; the caller rounds its float64 values. */
+typedef struct AtmoShadeParams {
+    float sun_global[3];
+    float solar_irradiance[3];
+    float view_right_global[3];
+    float view_up_global[3];
+    float view_forward_global[3];
+    int32_t advanced_ocean_enabled;
+    float ocean_roughness;
+    float ocean_f0;
+    float ocean_sky_strength;
+    float ocean_absorption[3];
+    float ocean_scattering[3];
+    float ocean_max_depth;
+    float ocean_refraction_index;
+    float ocean_refraction_strength;
+    float ocean_refraction_max_offset_pixels;
+    float sun_angular_radius;   /* radians */
+    float tangent_half_fov;
+    int32_t mode;               /* debug view: 0 lit, otherwise albedo */
+} AtmoShadeParams;
+
+/* Replaces `_shade_surface` (renderer.py:1277-1575) on a W x H frame;
+ * writes fields->surface_hdr for every pixel. */
+int32_t atmo_shade_surface(const AtmoShadeFields* fields, const AtmoShadeParams* params,
+                           int32_t width, int32_t height, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

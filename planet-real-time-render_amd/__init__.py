@@ -5,7 +5,9 @@ Drop-in for the reference's `planet_renderer.atmosphere` package
 `AtmosphereDiagnosticView`, `AtmosphereRenderer`, plus `LightingState`
 (src/planet_renderer/lighting.py), and the display post-process that
 follows the composite, `PostprocessConfig` / `PostProcessor`
-(src/planet_renderer/postprocess.py).  Compute runs in libatmo.so (HIP,
+(src/planet_renderer/postprocess.py), and the surface shading that consumes
+`prepare_frame`'s lighting, `shade_surface` (renderer.py:1277-1575).
+Compute runs in libatmo.so (HIP,
 gfx950); this package is its host side.
 """
 
@@ -15,6 +17,7 @@ from .frame_io import OCEAN_SURFACE_ID, check_gbuffer, load_hdr, save_hdr
 from .lighting import LightingState, StaticLightingProvider
 from .postprocess import PostprocessConfig, PostProcessor
 from .renderer import AtmosphereRenderer, DeviceField
+from .shading import SurfaceFields, shade_surface
 
 __all__ = [
     "AtmosphereConfig",
@@ -26,7 +29,9 @@ __all__ = [
     "PostProcessor",
     "PostprocessConfig",
     "StaticLightingProvider",
+    "SurfaceFields",
     "check_gbuffer",
     "load_hdr",
     "save_hdr",
+    "shade_surface",
 ]

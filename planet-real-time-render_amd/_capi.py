@@ -74,6 +74,7 @@ EXPORTED_SYMBOLS = (
     "atmo_post_get_buffer",
     "atmo_post_bind_buffer",
     "atmo_post_process",
+    "atmo_shade_surface",
 )
 
 POST_BUF_BLOOM_A = 0
@@ -145,6 +146,43 @@ class AtmoPostParams(ctypes.Structure):
         ("bloom_clamp", ctypes.c_double),
         ("bloom_strength", ctypes.c_double),
         ("bloom_passes", ctypes.c_int32),
+    ]
+
+
+SHADE_FIELD_NAMES = ("surface_id", "albedo", "normal", "position", "ocean_slope_variance",
+                     "water_depth_m", "seabed_albedo", "seabed_normal", "terrain_surface_id",
+                     "terrain_position", "terrain_albedo", "terrain_normal", "sun_transmittance",
+                     "sky_radiance", "surface_hdr")
+
+
+class AtmoShadeFields(ctypes.Structure):
+    """struct AtmoShadeFields (include/atmo.h): device pointers."""
+
+    _fields_ = [(name, ctypes.c_void_p) for name in SHADE_FIELD_NAMES]
+
+
+class AtmoShadeParams(ctypes.Structure):
+    """struct AtmoShadeParams (include/atmo.h), field for field."""
+
+    _fields_ = [
+        ("sun_global", ctypes.c_float * 3),
+        ("solar_irradiance", ctypes.c_float * 3),
+        ("view_right_global", ctypes.c_float * 3),
+        ("view_up_global", ctypes.c_float * 3),
+        ("view_forward_global", ctypes.c_float * 3),
+        ("advanced_ocean_enabled", ctypes.c_int32),
+        ("ocean_roughness", ctypes.c_float),
+        ("ocean_f0", ctypes.c_float),
+        ("ocean_sky_strength", ctypes.c_float),
+        ("ocean_absorption", ctypes.c_float * 3),
+        ("ocean_scattering", ctypes.c_float * 3),
+        ("ocean_max_depth", ctypes.c_float),
+        ("ocean_refraction_index", ctypes.c_float),
+        ("ocean_refraction_strength", ctypes.c_float),
+        ("ocean_refraction_max_offset_pixels", ctypes.c_float),
+        ("sun_angular_radius", ctypes.c_float),
+        ("tangent_half_fov", ctypes.c_float),
+        ("mode", ctypes.c_int32),
     ]
 
 
@@ -234,6 +272,8 @@ def _signatures(lib: ctypes.CDLL) -> None:
     lib.atmo_post_get_buffer.argtypes = [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int64)]
     lib.atmo_post_bind_buffer.argtypes = [_P, _I, _P, ctypes.c_int64]
     lib.atmo_post_process.argtypes = [_P, _P, _P, _F, _I, _I, _P]
+    lib.atmo_shade_surface.argtypes = [ctypes.POINTER(AtmoShadeFields),
+                                       ctypes.POINTER(AtmoShadeParams), _I, _I, _P]
     for name in EXPORTED_SYMBOLS:
         if name not in ("atmo_last_error", "atmo_mlp_last_error", "atmo_batch_last_error",
                         "atmo_post_last_error"):
@@ -286,6 +326,8 @@ def check(status: int, handle=None) -> None:
 __all__ = [
     "AtmoBatchState",
     "AtmoPostParams",
+    "AtmoShadeFields",
+    "AtmoShadeParams",
     "AtmoParams",
     "AtmoError",
     "EXPORTED_SYMBOLS",

@@ -29,7 +29,8 @@ CSRC = PACKAGE_DIR / "csrc"
 # instruction-throughput probe the benchmark uses for its roofline peak.
 TARGETS = {
     PACKAGE_DIR / "libatmo.so": [CSRC / "atmo_kernels.hip", CSRC / "atmo_robust.hip",
-                                 CSRC / "surrogate.hip", CSRC / "postprocess.hip"],
+                                 CSRC / "surrogate.hip", CSRC / "postprocess.hip",
+                                 CSRC / "shade.hip"],
     PACKAGE_DIR / "libatmo_probe.so": [CSRC / "probe.hip"],
 }
 ARCH = "gfx950"

@@ -331,3 +331,109 @@ __all__ = [
     "sun_from_angles",
     "view_basis_local",
 ]
+
+
+# --------------------------------------------------------------- shading
+
+OCEAN_SURFACE_ID = 2_147_000_000  # src/planet_renderer/ocean.py:17
+# OceanConfig defaults (src/planet_renderer/ocean.py:46-65)
+OCEAN_DEFAULTS = {
+    "ocean_roughness": 0.12,
+    "ocean_f0": 0.02,
+    "ocean_sky_strength": 1.0,
+    "ocean_absorption": (0.14, 0.055, 0.022),
+    "ocean_scattering": (0.0025, 0.005, 0.012),
+    "ocean_max_depth": 80.0,
+    "ocean_refraction_index": 1.333,
+    "ocean_refraction_strength": 1.0,
+    "ocean_refraction_max_offset_pixels": 48.0,
+}
+
+
+@dataclass
+class ShadingInputs:
+    """The fields and scalars `_shade_surface` consumes
+    (src/planet_renderer/renderer.py:1277-1300), host-side (NumPy).
+    `fields` holds the G-buffer, the pre-ocean terrain snapshot and the
+    surface-lighting outputs under the AtmoShadeFields names; `params` the
+    scalar arguments under the AtmoShadeParams names."""
+
+    width: int
+    height: int
+    fields: dict
+    params: dict
+
+
+def make_shading_inputs(width: int = 1920, height: int = 1080, seed_offset: int = 0,
+                        water_plane_m: float = 40.0) -> ShadingInputs:
+    """A seeded G-buffer with terrain and ocean pixels.  The ocean is a
+    water plane `water_plane_m` below the camera (view coordinates) over a
+    smoothly varying seabed 1-70 m deeper, so the refracted seabed lookup
+    finds terrain snapshot samples behind the water surface; terrain pixels
+    keep the frame generator's positions.  Sky pixels are as make_frame
+    draws them.  Local and global frames coincide."""
+    frame = make_frame(width=width, height=height, config_index=1, seed_offset=seed_offset)
+    g = np.random.Generator(np.random.PCG64(SEED_BASE + 7 + 1000 * seed_offset))
+    right, up, forward = frame.view_basis
+    rays = pixel_rays_view(width, height, frame.tangent_half_fov)
+    sid = frame.surface_id.copy()
+    surface = sid >= 0
+    sid[surface] = g.integers(0, 1 << 20, size=int(surface.sum()), dtype=np.int32)
+    below = rays[:, :, 1] < -0.02
+    ocean = surface & below & (g.random((width, height)) < 0.95)
+    sid[ocean] = OCEAN_SURFACE_ID
+    pos = frame.position_view.astype(np.float64)
+    t_water = water_plane_m / np.maximum(-rays[:, :, 1], 1e-6)
+    pos[ocean] = rays[ocean] * t_water[ocean, None]
+    xs = np.arange(width)[:, None] / width
+    ys = np.arange(height)[None, :] / height
+    depth = 1.0 + 69.0 * (0.5 + 0.5 * np.sin(9.0 * xs + 4.0 * ys) * np.cos(5.0 * ys - 3.0 * xs))
+    t_bed = (water_plane_m + depth) / np.maximum(-rays[:, :, 1], 1e-6)
+
+    def unit(v):
+        return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+    world_up = np.array([0.0, 1.0, 0.0])
+    normal = unit(world_up + 0.35 * g.normal(size=(width, height, 3)))
+    normal[ocean] = unit(up + 0.06 * g.normal(size=(int(ocean.sum()), 3)))
+    albedo = g.uniform(0.02, 0.8, (width, height, 3))
+    albedo[ocean] = (0.012, 0.055, 0.16)
+    terrain_sid = sid.copy()
+    terrain_sid[ocean] = 0
+    terrain_sid[ocean & (g.random((width, height)) < 0.05)] = -1  # holes in the snapshot
+    terrain_pos = pos.copy()
+    terrain_pos[ocean] = rays[ocean] * t_bed[ocean, None]
+    terrain_albedo = albedo.copy()
+    terrain_albedo[ocean] = g.uniform(0.1, 0.6, (int(ocean.sum()), 3))
+    terrain_normal = normal.copy()
+    terrain_normal[ocean] = unit(world_up + 0.3 * g.normal(size=(int(ocean.sum()), 3)))
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+    fields = {
+        "surface_id": np.ascontiguousarray(sid, dtype=np.int32),
+        "albedo": f32(albedo),
+        "normal": f32(normal),
+        "position": f32(pos),
+        "ocean_slope_variance": f32(g.uniform(0.0, 0.02, (width, height))),
+        "water_depth_m": f32(np.clip(depth / np.maximum(-rays[:, :, 1], 0.05), 0.0, 500.0)),
+        "seabed_albedo": f32(g.uniform(0.1, 0.6, (width, height, 3))),
+        "seabed_normal": f32(unit(world_up + 0.3 * g.normal(size=(width, height, 3)))),
+        "terrain_surface_id": np.ascontiguousarray(terrain_sid, dtype=np.int32),
+        "terrain_position": f32(terrain_pos),
+        "terrain_albedo": f32(terrain_albedo),
+        "terrain_normal": f32(terrain_normal),
+        "sun_transmittance": f32(g.uniform(0.0, 1.0, (width, height, 3))),
+        "sky_radiance": f32(g.uniform(0.0, 0.3, (width, height, 3))),
+    }
+    params = {
+        "sun_global": tuple(frame.sun_local),
+        "solar_irradiance": SOLAR_IRRADIANCE,
+        "view_right_global": tuple(right),
+        "view_up_global": tuple(up),
+        "view_forward_global": tuple(forward),
+        "advanced_ocean_enabled": 1,
+        **OCEAN_DEFAULTS,
+        "sun_angular_radius": math.radians(SUN_ANGULAR_RADIUS_DEG),
+        "tangent_half_fov": frame.tangent_half_fov,
+        "mode": 0,
+    }
+    return ShadingInputs(width=width, height=height, fields=fields, params=params)
