@@ -10,8 +10,9 @@ The frame is timed twice: with the plain composite and with the learned
 surrogate composite.  Both are checked against the float64 CPU
 restatement; `value` is the faster form whose p99 relative error is within
 1 % (the plain form always is).  The 256-state batched LUT build is timed
-after the frames, and the display post-process (bloom, ACES, sRGB) of the
-plain frame's HDR output after that.
+after the frames, then the display post-process (bloom, ACES, sRGB) of the
+plain frame's HDR output and the surface shading of a synthetic 1080p
+G-buffer.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -300,6 +301,78 @@ def bench_postprocess(pkg, hdr, device, steps: int, with_oracle: bool):
     return out
 
 
+# Surface-shading algorithmic bytes per pixel: surface_id (4 B) and the
+# output (12 B) for every pixel; albedo, normal, sun transmittance and sky
+# radiance (48 B) on surface pixels; position, slope variance, water depth,
+# seabed albedo and normal (36 B) and the terrain snapshot's id, position,
+# albedo and normal (40 B, the lookup's unique footprint) on ocean pixels.
+SHADE_BYTES = {"sky": 16, "surface": 64, "ocean": 64 + 36 + 40}
+
+
+def bench_shading(pkg, scene, device, steps: int, with_oracle: bool):
+    """_shade_surface (renderer.py:1277-1575) on a synthetic 1080p G-buffer
+    with terrain and ocean pixels, timed with events around `steps` calls."""
+    import torch
+
+    inputs = scene.make_shading_inputs(1920, 1080, seed_offset=0)
+    f = inputs.fields
+    sf = pkg.SurfaceFields.allocate(1920, 1080, device)
+    names = {"gbuffer_surface_id": "surface_id", "gbuffer_albedo": "albedo",
+             "gbuffer_normal": "normal", "gbuffer_position": "position",
+             "gbuffer_ocean_slope_variance": "ocean_slope_variance",
+             "gbuffer_water_depth_m": "water_depth_m", "gbuffer_seabed_albedo": "seabed_albedo",
+             "gbuffer_seabed_normal": "seabed_normal", "terrain_surface_id": "terrain_surface_id",
+             "terrain_position": "terrain_position", "terrain_albedo": "terrain_albedo",
+             "terrain_normal": "terrain_normal"}
+    for name, key in names.items():
+        getattr(sf, name).tensor.copy_(torch.from_numpy(f[key]))
+    sun_t = torch.from_numpy(f["sun_transmittance"]).to(device)
+    sky = torch.from_numpy(f["sky_radiance"]).to(device)
+    p = inputs.params
+    args = (p["view_right_global"], p["view_up_global"], p["view_forward_global"],
+            p["advanced_ocean_enabled"], p["ocean_roughness"], p["ocean_f0"],
+            p["ocean_sky_strength"], p["ocean_absorption"], p["ocean_scattering"],
+            p["ocean_max_depth"], p["ocean_refraction_index"], p["ocean_refraction_strength"],
+            p["ocean_refraction_max_offset_pixels"], p["sun_angular_radius"],
+            p["tangent_half_fov"], p["mode"])
+
+    def call():
+        pkg.shade_surface(sf, p["sun_global"], p["solar_irradiance"], sun_t, sky, *args)
+
+    stream = torch.cuda.current_stream(device)
+    for _ in range(5):
+        call()
+    torch.cuda.synchronize(device)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        call()
+    e1.record(stream)
+    torch.cuda.synchronize(device)
+    us = e0.elapsed_time(e1) / steps * 1e3
+    sid = f["surface_id"]
+    n_sky = int((sid < 0).sum())
+    n_ocean = int((sid == scene.OCEAN_SURFACE_ID).sum())
+    n_surface = sid.size - n_sky - n_ocean
+    nbytes = (SHADE_BYTES["sky"] * n_sky + SHADE_BYTES["surface"] * n_surface
+              + SHADE_BYTES["ocean"] * n_ocean)
+    out = {"workload": "1920x1080 synthetic G-buffer: terrain, ocean (water plane over a "
+                       "1-70 m seabed) and sky pixels; OceanConfig defaults",
+           "pixels": {"sky": n_sky, "terrain": n_surface, "ocean": n_ocean},
+           "us": round(us, 2), "algorithmic_bytes": nbytes,
+           "hbm_gbs": round(nbytes / (us * 1e-6) / 1e9, 1),
+           "hbm_frac": round(nbytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+    if with_oracle:
+        from oracle.surface_cpu import shade
+
+        got = sf.surface_hdr.to_numpy()
+        c0 = time.perf_counter()
+        want, _ = shade(f, p, 1920, 1080)
+        out["cpu_oracle_ms"] = round((time.perf_counter() - c0) * 1e3, 2)
+        out["values_differing_from_oracle"] = int((got.view(np.uint32) != want.view(np.uint32)).sum())
+    return out
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -384,6 +457,11 @@ def main() -> int:
         f"{postprocess['resolve_only']['us']:.2f} us"
         + (f"; {postprocess['display_values_differing_from_oracle']} display values differ "
            f"from the oracle" if "cpu_oracle_ms" in postprocess else ""))
+
+    shading = bench_shading(pkg, scene, device, args.steps, not args.no_cpu_baseline)
+    log(f"surface shading: {shading['us']:.2f} us ({shading['hbm_gbs']:.0f} GB/s algorithmic)"
+        + (f"; {shading['values_differing_from_oracle']} values differ from the oracle"
+           if "cpu_oracle_ms" in shading else ""))
 
     n_pix = inputs.width * inputs.height
     surf_mask = inputs.surface_id >= 0
@@ -535,6 +613,7 @@ def main() -> int:
         "parity": parity,
         "batch": batch,
         "postprocess": postprocess,
+        "shading": shading,
     }
     frame.r.close()
     print(json.dumps(result), flush=True)
