@@ -46,7 +46,8 @@ def gpu_process(pkg, hdr, config, ev, bloom, tone_map, device):
     post.process(src, display, ev, bloom=bloom, tone_map=tone_map)
     torch.cuda.synchronize(device)
     out = (display.cpu().numpy(), post.bloom_a.to_numpy(), post.bloom_b.to_numpy())
-    assert np.array_equal(src.cpu().numpy(), hdr), "process modified its HDR source"
+    assert np.array_equal(src.cpu().numpy().view(np.uint32), hdr.view(np.uint32)), \
+        "process modified its HDR source"
     post.close()
     return out
 
@@ -68,6 +69,7 @@ def assert_bit_exact(name, got, want):
 
 
 CASES = [
+    ((3840, 2160), {}, 0.0, True, True),
     ((1920, 1080), {}, 0.0, True, True),
     ((97, 61), {}, 1.0, True, True),
     ((64, 64), {"bloom_passes": 1}, -2.0, True, True),
@@ -97,6 +99,28 @@ def test_post_process_bit_exact(pkg, device, size, changes, ev, bloom, tone_map)
     assert_bit_exact("display", got[0], want[0])
     assert_bit_exact("bloom_a", got[1], want[1])
     assert_bit_exact("bloom_b", got[2], want[2])
+
+
+def test_post_process_non_finite_radiance(pkg, device):
+    """NaN and infinite HDR pixels go through the same IEEE operations on
+    both sides (maxNum drops NaN, inf / inf is NaN, the clamp maps NaN to
+    0): the outputs agree value for value, NaN for NaN."""
+    hdr = hdr_frame(130, 70, seed=3)
+    hdr[10, 10] = np.nan
+    hdr[50, 20, 1] = np.inf
+    hdr[90, 40] = (np.inf, -np.inf, np.nan)
+    hdr[129, 69, 2] = np.nan
+    config = pkg.PostprocessConfig()
+    for bloom, tone_map in ((True, True), (False, True), (False, False), (True, False)):
+        got = gpu_process(pkg, hdr, config, 0.5, bloom, tone_map, device)
+        want = oracle_process(hdr, config, 0.5, bloom, tone_map)
+        for name, g, w in zip(("display", "bloom_a", "bloom_b"), got, want):
+            if not bloom and name != "display":
+                continue
+            nan_g, nan_w = np.isnan(g), np.isnan(w)
+            assert (nan_g == nan_w).all(), f"{name}: NaN positions differ"
+            assert_bit_exact(name, np.where(nan_g, 0, g).astype(np.float32),
+                             np.where(nan_w, 0, w).astype(np.float32))
 
 
 def test_post_process_of_a_composite_frame(pkg, scene, device, tmp_path):

@@ -62,6 +62,7 @@ def assert_bit_exact(got, want):
 
 
 CASES = [
+    ((3840, 2160), 9, {}),
     ((1920, 1080), 0, {}),
     ((321, 179), 1, {}),
     ((321, 179), 2, {"ocean_refraction_max_offset_pixels": 3.0}),
