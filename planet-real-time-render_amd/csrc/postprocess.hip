@@ -6,18 +6,18 @@
 // downsample + bright pass into a half-resolution field), P passes of
 // `_blur_horizontal` + `_blur_vertical`, and `_resolve` (exposure,
 // bilinear bloom, ACES + sRGB or a [0, 1] clamp).  Here three kernels:
-//  - k_post_extract: the bright pass, one half-resolution texel per lane,
-//    into a handle-owned field;
-//  - k_post_blur: all P pass pairs in LDS.  A 32x32 tile of that field
-//    loads its halo of 2P texels (every load issued before the first LDS
-//    store) and runs the passes ping-pong between two LDS copies, each
-//    pass over the shrinking region the next one reads.  A lane produces 4
-//    consecutive outputs along the pass axis from a sliding window of 8.
-//    Tiles whose halo lies inside the frame (82% at 1080p) run with
-//    compile-time regions and no clamping; the others clamp every window
-//    position as the reference clamps its taps.  The tile's interior goes
-//    to bloom_a (last vertical pass) and bloom_b (last horizontal pass),
-//    the fields the reference leaves;
+//  - k_post_blur: the bright pass and all P pass pairs, one 48x48 tile of
+//    half-resolution texels per 1024-thread block.  The block extracts its
+//    tile plus a halo of 2P texels from the HDR frame into LDS (each
+//    lane's hdr loads issued before its arithmetic), then runs the passes
+//    ping-pong between two LDS copies (100 KB at P = 4, 156 KB at P = 8),
+//    each pass over the shrinking region the next one reads.  A lane
+//    produces 4 consecutive outputs along the pass axis from a sliding
+//    window of 8.  Tiles whose halo lies inside the frame (180 of 240 at
+//    1080p) run with compile-time regions and no clamping; the others
+//    clamp every window position as the reference clamps its taps.  The
+//    tile's interior goes to bloom_a (last vertical pass) and bloom_b (last
+//    horizontal pass), the fields the reference leaves;
 //  - k_post_resolve_bloom: four consecutive pixels per lane, 16-byte loads
 //    and stores, the bloom taps of all four pixels loaded before any
 //    arithmetic; without bloom every float of the frame is independent and
@@ -45,9 +45,9 @@
 namespace {
 
 constexpr uint32_t kPostMagic = 0x504f5354u;  // "POST"
-constexpr int kTile = 32;                     // half-resolution texels per tile side
+constexpr int kTile = 48;                     // half-resolution texels per tile side
 constexpr int kThreads = 256;
-constexpr int kBlurThreads = 512;
+constexpr int kBlurThreads = 1024;
 constexpr int kXcds = 8;
 
 struct PostArgs {
@@ -55,7 +55,6 @@ struct PostArgs {
     float* display;
     float* bloom_a;
     float* bloom_b;
-    float* extracted;
     int32_t width, height, half_width, half_height;
     float width_f, height_f, half_width_f, half_height_f;
     double inv_height;
@@ -67,36 +66,6 @@ struct PostArgs {
 
 __device__ __forceinline__ float output_channel(float value, int32_t tone_map) {
     return tone_map ? post::display_channel(value) : fminf(fmaxf(value, 0.0f), 1.0f);
-}
-
-// _extract (postprocess.py:110-119): one half-resolution texel per lane,
-// blocks of 4 columns x 64 rows, lanes along y (the contiguous axis).
-__global__ __launch_bounds__(256) void k_post_extract(PostArgs a) {
-    const int gy = (int)blockIdx.x * 64 + ((int)threadIdx.x & 63);
-    const int gx = (int)blockIdx.y * 4 + ((int)threadIdx.x >> 6);
-    if (gy >= a.half_height || gx >= a.half_width) return;
-    float r = 0.0f, g = 0.0f, b = 0.0f;
-#pragma unroll
-    for (int ox = 0; ox < 2; ++ox) {
-        const int sx = min(gx * 2 + ox, a.width - 1);
-#pragma unroll
-        for (int oy = 0; oy < 2; ++oy) {
-            const int sy = min(gy * 2 + oy, a.height - 1);
-            const float* p = a.hdr + ((size_t)sx * a.height + sy) * 3;
-            r += p[0];
-            g += p[1];
-            b += p[2];
-        }
-    }
-    const float s = 0.25f * a.exposure_multiplier;
-    r *= s;
-    g *= s;
-    b *= s;
-    post::soft_threshold(a.bright, r, g, b);
-    float* o = a.extracted + ((size_t)gx * a.half_height + gy) * 3;
-    o[0] = r;
-    o[1] = g;
-    o[2] = b;
 }
 
 // One blur pass over the local region [lo0, hi0) x [lo1, hi1) of the tile,
@@ -193,9 +162,9 @@ __device__ __forceinline__ void interior_passes(float* A, float* B) {
     }
 }
 
-// P x (_blur_horizontal, _blur_vertical) on one 32x32 tile of the
-// extracted field with its halo of 2P texels, in LDS; writes the tile's
-// bloom_a (last vertical pass) and bloom_b (last horizontal pass).
+// _extract, then P x (_blur_horizontal, _blur_vertical), on one 48x48 tile
+// with its halo of 2P texels, in LDS; writes the tile's bloom_a (last
+// vertical pass) and bloom_b (last horizontal pass).
 template <int P>
 __global__ __launch_bounds__(kBlurThreads) void k_post_blur(PostArgs a) {
     constexpr int R = 2 * P;
@@ -213,31 +182,58 @@ __global__ __launch_bounds__(kBlurThreads) void k_post_blur(PostArgs a) {
     const int gx0 = X0 - R, gy0 = Y0 - R;
     const int hw = a.half_width, hh = a.half_height;
 
-    // every global load of the halo tile is issued before the first LDS store
+    // _extract (postprocess.py:110-119) over the whole local region, in
+    // chunks of CH texels per lane whose hdr loads are all issued before
+    // the first arithmetic; texels outside the frame are zero and never read
     constexpr int NL = (L * L + kBlurThreads - 1) / kBlurThreads;
-    float v[NL][3];
+    constexpr int CH = 4;
 #pragma unroll
-    for (int k = 0; k < NL; ++k) {
-        const int i = (int)threadIdx.x + k * kBlurThreads;
-        const int lx = i / L, ly = i - lx * L;
-        const int gx = gx0 + lx, gy = gy0 + ly;
-        v[k][0] = v[k][1] = v[k][2] = 0.0f;
-        if (i < L * L && gx >= 0 && gx < hw && gy >= 0 && gy < hh) {
-            const float* p = a.extracted + ((size_t)gx * hh + gy) * 3;
-            v[k][0] = p[0];
-            v[k][1] = p[1];
-            v[k][2] = p[2];
+    for (int k0 = 0; k0 < NL; k0 += CH) {
+        float px[CH][12];
+        bool in[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const int i = (int)threadIdx.x + (k0 + c) * kBlurThreads;
+            const int lx = i / L, ly = i - lx * L;
+            const int gx = gx0 + lx, gy = gy0 + ly;
+            in[c] = k0 + c < NL && i < L * L && gx >= 0 && gx < hw && gy >= 0 && gy < hh;
+            if (in[c]) {
+#pragma unroll
+                for (int ox = 0; ox < 2; ++ox) {
+                    const int sx = min(gx * 2 + ox, a.width - 1);
+#pragma unroll
+                    for (int oy = 0; oy < 2; ++oy) {
+                        const int sy = min(gy * 2 + oy, a.height - 1);
+                        const float* p = a.hdr + ((size_t)sx * a.height + sy) * 3;
+#pragma unroll
+                        for (int ch = 0; ch < 3; ++ch) px[c][(ox * 2 + oy) * 3 + ch] = p[ch];
+                    }
+                }
+            }
         }
-    }
 #pragma unroll
-    for (int k = 0; k < NL; ++k) {
-        const int i = (int)threadIdx.x + k * kBlurThreads;
-        if (i < L * L) {
+        for (int c = 0; c < CH; ++c) {
+            const int i = (int)threadIdx.x + (k0 + c) * kBlurThreads;
+            if (k0 + c >= NL || i >= L * L) continue;
+            float r = 0.0f, g = 0.0f, b = 0.0f;
+            if (in[c]) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    r += px[c][3 * t];
+                    g += px[c][3 * t + 1];
+                    b += px[c][3 * t + 2];
+                }
+                const float s = 0.25f * a.exposure_multiplier;
+                r *= s;
+                g *= s;
+                b *= s;
+                post::soft_threshold(a.bright, r, g, b);
+            }
             const int lx = i / L, ly = i - lx * L;
             const int li = lx * LP + ly;
-            A[li] = v[k][0];
-            A[N + li] = v[k][1];
-            A[2 * N + li] = v[k][2];
+            A[li] = r;
+            A[N + li] = g;
+            A[2 * N + li] = b;
         }
     }
     __syncthreads();
@@ -411,7 +407,6 @@ struct AtmoPost {
     float bloom_strength;
     float* bloom[2];
     bool borrowed[2];
-    float* extracted;  // _extract's output, read by the blur tiles
     char error[256];
 };
 
@@ -500,11 +495,11 @@ int32_t atmo_post_create(const AtmoPostParams* params, int32_t width, int32_t he
     h->bloom_strength = (float)params->bloom_strength;
     const size_t bytes = bloom_floats(h) * sizeof(float);
     bool ok = true;
-    for (float** p : {&h->bloom[0], &h->bloom[1], &h->extracted})
+    for (float** p : {&h->bloom[0], &h->bloom[1]})
         ok = ok && hipMalloc(p, bytes) == hipSuccess && hipMemset(*p, 0, bytes) == hipSuccess;
     if (!ok) {
         post_error(nullptr, "atmo_post_create: device allocation failed");
-        for (float* p : {h->bloom[0], h->bloom[1], h->extracted})
+        for (float* p : {h->bloom[0], h->bloom[1]})
             if (p) (void)hipFree(p);
         h->magic = 0;
         delete h;
@@ -519,7 +514,6 @@ int32_t atmo_post_destroy(AtmoPost* h) {
     (void)hipDeviceSynchronize();
     for (int i = 0; i < 2; ++i)
         if (h->bloom[i] && !h->borrowed[i]) (void)hipFree(h->bloom[i]);
-    if (h->extracted) (void)hipFree(h->extracted);
     h->magic = 0;
     delete h;
     return ATMO_OK;
@@ -594,13 +588,10 @@ int32_t atmo_post_process(AtmoPost* h, const float* hdr, float* display,
     a.bloom_strength = h->bloom_strength;
     a.bright = h->bright;
     a.tone_map = tone_map != 0;
-    a.extracted = h->extracted;
     a.inv_height = 1.0 / (double)h->height;
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = hipSuccess;
     if (use_bloom) {
-        hipLaunchKernelGGL(k_post_extract, dim3((h->half_height + 63) / 64, (h->half_width + 3) / 4),
-                           dim3(256), 0, s, a);
         a.tiles_x = (h->half_width + kTile - 1) / kTile;
         a.tiles_y = (h->half_height + kTile - 1) / kTile;
         a.tiles_per_xcd = (a.tiles_x * a.tiles_y + kXcds - 1) / kXcds;
