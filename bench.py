@@ -11,8 +11,9 @@ surrogate composite.  Both are checked against the float64 CPU
 restatement; `value` is the faster form whose p99 relative error is within
 1 % (the plain form always is).  The 256-state batched LUT build is timed
 after the frames, then the display post-process (bloom, ACES, sRGB) of the
-plain frame's HDR output and the surface shading of a synthetic 1080p
-G-buffer.
+plain frame's HDR output, the surface shading of a synthetic 1080p
+G-buffer, and the whole atmosphere frame of the reference's render order
+(LUTs, surface lighting, shading, composite, post-process).
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -373,6 +374,56 @@ def bench_shading(pkg, scene, device, steps: int, with_oracle: bool):
     return out
 
 
+def bench_full_frame(pkg, scene, device, steps: int, warmup: int, stream):
+    """The atmosphere part of a whole reference frame (renderer.py:1664-1735):
+    the cold frame's T, MS and SV rebuilds, surface lighting, aerial
+    perspective, then _shade_surface, the composite of its output, and the
+    display post-process, on a 1080p G-buffer with terrain and ocean."""
+    import torch
+
+    shading = scene.make_shading_inputs(1920, 1080, seed_offset=0)
+    inputs = scene.make_frame(width=1920, height=1080, config_index=1, seed_offset=0)
+    inputs.position_view = shading.fields["position"]
+    inputs.surface_id = shading.fields["surface_id"]
+    frame = PlainFrame(pkg, inputs, device)
+    fields = pkg.SurfaceFields.allocate(1920, 1080, device)
+    for name in ("surface_id", "albedo", "normal", "position", "ocean_slope_variance",
+                 "water_depth_m", "seabed_albedo", "seabed_normal"):
+        getattr(fields, "gbuffer_" + name).tensor.copy_(torch.from_numpy(shading.fields[name]))
+    for name in ("surface_id", "position", "albedo", "normal"):
+        getattr(fields, "terrain_" + name).tensor.copy_(
+            torch.from_numpy(shading.fields["terrain_" + name]))
+    post = pkg.PostProcessor(1920, 1080, pkg.PostprocessConfig(), device)
+    display = torch.empty_like(frame.hdr)
+    p = shading.params
+    r = frame.r
+    right, up, forward = frame.basis
+    c0, c1 = inputs.star_contrast
+    stages = frame.stages()[:-1] + [
+        ("surface_shading", lambda: pkg.shade_surface(
+            fields, p["sun_global"], p["solar_irradiance"], r.surface_sun_transmittance,
+            r.surface_sky_radiance, p["view_right_global"], p["view_up_global"],
+            p["view_forward_global"], p["advanced_ocean_enabled"], p["ocean_roughness"],
+            p["ocean_f0"], p["ocean_sky_strength"], p["ocean_absorption"], p["ocean_scattering"],
+            p["ocean_max_depth"], p["ocean_refraction_index"], p["ocean_refraction_strength"],
+            p["ocean_refraction_max_offset_pixels"], p["sun_angular_radius"],
+            p["tangent_half_fov"], p["mode"])),
+        ("composite", lambda: r._composite(
+            fields.surface_hdr.tensor, frame.space, frame.sid, frame.pos, frame.hdr, frame.R,
+            frame.cam_alt, frame.alt_key, frame.sun, frame.E, frame.disk, frame.ar, right, up,
+            forward, inputs.tangent_half_fov, c0, c1, 0)),
+        ("postprocess", lambda: post.process(frame.hdr, display, 0.0, bloom=True, tone_map=True)),
+    ]
+    wall, frame_ms, stage_ms = time_frames(stages, steps, warmup, stream, device)
+    post.close()
+    r.close()
+    return {"workload": "1920x1080 frame on a G-buffer with terrain and ocean pixels: T, MS, SV "
+                        "rebuilt, surface lighting, aerial perspective, surface shading, "
+                        "composite, post-process (bloom + ACES/sRGB)",
+            "wall_ms": round(wall, 4), "frame_ms_median": round(float(np.median(frame_ms)), 4),
+            "stage_us": {k: round(v * 1e3, 2) for k, v in stage_ms.items()}}
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -462,6 +513,10 @@ def main() -> int:
     log(f"surface shading: {shading['us']:.2f} us ({shading['hbm_gbs']:.0f} GB/s algorithmic)"
         + (f"; {shading['values_differing_from_oracle']} values differ from the oracle"
            if "cpu_oracle_ms" in shading else ""))
+
+    full_frame = bench_full_frame(pkg, scene, device, args.steps, args.warmup, stream)
+    log(f"full atmosphere frame with shading and post-process: {full_frame['wall_ms']:.4f} "
+        "ms/frame (wall)")
 
     n_pix = inputs.width * inputs.height
     surf_mask = inputs.surface_id >= 0
@@ -614,6 +669,7 @@ def main() -> int:
         "batch": batch,
         "postprocess": postprocess,
         "shading": shading,
+        "full_frame": full_frame,
     }
     frame.r.close()
     print(json.dumps(result), flush=True)

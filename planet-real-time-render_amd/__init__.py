@@ -15,6 +15,7 @@ from .config import AtmosphereConfig
 from .diagnostics import AtmosphereDiagnosticView
 from .frame_io import OCEAN_SURFACE_ID, check_gbuffer, load_hdr, save_hdr
 from .lighting import LightingState, StaticLightingProvider
+from .pipeline import OceanShading, render_frame
 from .postprocess import PostprocessConfig, PostProcessor
 from .renderer import AtmosphereRenderer, DeviceField
 from .shading import SurfaceFields, shade_surface
@@ -26,12 +27,14 @@ __all__ = [
     "DeviceField",
     "LightingState",
     "OCEAN_SURFACE_ID",
+    "OceanShading",
     "PostProcessor",
     "PostprocessConfig",
     "StaticLightingProvider",
     "SurfaceFields",
     "check_gbuffer",
     "load_hdr",
+    "render_frame",
     "save_hdr",
     "shade_surface",
 ]

@@ -1,0 +1,104 @@
+"""The atmosphere part of a reference frame end to end
+(src/planet_renderer/renderer.py:1664-1735): update -> prepare_frame ->
+_shade_surface -> composite -> PostProcessor.process, through
+`pipeline.render_frame`, against the same chain of CPU oracles
+(oracle/ref_cpu.c, surface_cpu.c, post_cpu.c).
+
+Bar: the atmosphere stages differ from the float32 oracle by float32
+rounding of transcendentals (at most a few 1e-6 relative), and the
+shading, composite and post-process carry that through; the display is
+compared at 1e-4 relative with a 1e-6 floor, the composite's HDR at the
+1e-4 parity bar."""
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def device():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch.device("cuda:0")
+
+
+def oracle_chain(scene, inputs, shading, config_post, exposure_ev):
+    from oracle.post_cpu import PostOracle
+    from oracle.ref_cpu import RefCpu
+    from oracle.surface_cpu import shade
+
+    ref = RefCpu("f32")
+    first = ref.frame(inputs)  # LUTs and the surface lighting
+    fields = dict(shading.fields)
+    fields["sun_transmittance"] = first["surface_sun_transmittance"]
+    fields["sky_radiance"] = first["surface_sky_radiance"]
+    surface_hdr, _ = shade(fields, shading.params, inputs.width, inputs.height)
+    R, r_cam = inputs.planet_radius_m, inputs.camera_radius_m
+    light = inputs.lighting
+    alt_key = float(np.float32(max(r_cam - R, 0.0)))
+    right, up, forward = inputs.view_basis
+    c0, c1 = inputs.star_contrast
+    hdr, _ = ref.composite(inputs.config, surface_hdr, inputs.space_hdr, inputs.surface_id,
+                           inputs.position_view, R, max(r_cam - R, 0.0), alt_key,
+                           np.asarray(inputs.sun_local, np.float64),
+                           tuple(float(v) for v in light.solar_irradiance),
+                           light.sun_disk_radiance, math.radians(light.sun_angular_radius_degrees),
+                           right, up, forward, inputs.tangent_half_fov, c0, c1, 0,
+                           first["transmittance"], first["multi_scattering"], first["sky_view"],
+                           first["aerial"])
+    display = PostOracle(inputs.width, inputs.height, config_post).process(
+        hdr, exposure_ev, bloom=True, tone_map=True)
+    return surface_hdr, hdr, display
+
+
+@pytest.mark.parametrize("size, seed", [((320, 180), 0), ((1920, 1080), 1)])
+def test_render_frame_matches_the_oracle_chain(pkg, scene, device, size, seed):
+    import torch
+
+    w, h = size
+    shading = scene.make_shading_inputs(w, h, seed_offset=seed)
+    inputs = scene.make_frame(width=w, height=h, config_index=1, seed_offset=seed)
+    # the G-buffer the shading inputs hold (ocean pixels re-positioned on the water plane)
+    inputs.position_view = shading.fields["position"]
+    inputs.surface_id = shading.fields["surface_id"]
+    print(f"render_frame {size} seed {seed}", flush=True)
+    atmosphere = pkg.AtmosphereRenderer(w, h, inputs.config, device)
+    config_post = pkg.PostprocessConfig(exposure_ev=0.5)
+    post = pkg.PostProcessor(w, h, config_post, device)
+    fields = pkg.SurfaceFields.allocate(w, h, device)
+    for name in ("surface_id", "albedo", "normal", "position", "ocean_slope_variance",
+                 "water_depth_m", "seabed_albedo", "seabed_normal"):
+        getattr(fields, "gbuffer_" + name).tensor.copy_(torch.from_numpy(shading.fields[name]))
+    for name in ("surface_id", "position", "albedo", "normal"):
+        getattr(fields, "terrain_" + name).tensor.copy_(
+            torch.from_numpy(shading.fields["terrain_" + name]))
+    space = torch.from_numpy(inputs.space_hdr).to(device)
+    hdr = torch.full((w, h, 3), float("nan"), device=device)
+    display = torch.full((w, h, 3), float("nan"), device=device)
+    pkg.render_frame(atmosphere, post, fields, space, hdr, display,
+                     planet_radius_m=inputs.planet_radius_m, camera_radius_m=inputs.camera_radius_m,
+                     sun_local=inputs.sun_local, lighting=inputs.lighting,
+                     view_basis=inputs.view_basis, tangent_half_fov=inputs.tangent_half_fov,
+                     star_contrast=inputs.star_contrast, exposure_ev=config_post.exposure_ev)
+    torch.cuda.synchronize(device)
+    got_surface = fields.surface_hdr.to_numpy()
+    got_hdr, got_display = hdr.cpu().numpy(), display.cpu().numpy()
+    atmosphere.close()
+    post.close()
+    want_surface, want_hdr, want_display = oracle_chain(scene, inputs, shading, config_post,
+                                                        config_post.exposure_ev)
+
+    def rel(got, want):
+        floor = 1.0e-6 * np.abs(want).reshape(-1, 3).max(axis=0)
+        return np.abs(got.astype(np.float64) - want) / np.maximum(np.abs(want), floor)
+
+    for name, g, wv in (("surface_hdr", got_surface, want_surface), ("hdr", got_hdr, want_hdr),
+                        ("display", got_display, want_display)):
+        assert np.isfinite(g).all(), name
+        worst = float(rel(g, wv).max())
+        print(f"  {name}: max rel err {worst:.2e}", flush=True)
+        assert worst <= 1.0e-4, f"{name}: {worst:.3e}"
