@@ -178,7 +178,7 @@ def time_frames(stages, steps: int, warmup: int, stream, device):
     """Warm up, then time `steps` frames back to back: wall ms/frame between
     two device synchronisations.  A second pass of `steps` frames records
     HIP events between the stages for the per-frame durations and per-stage
-    means (the event records themselves add a few microseconds per stage,
+    medians (the event records themselves add a few microseconds per stage,
     so they stay out of the wall-clock pass)."""
     import torch
 
@@ -204,8 +204,10 @@ def time_frames(stages, steps: int, warmup: int, stream, device):
             ev[j + 1].record(stream)
     torch.cuda.synchronize(device)
     frame_ms = np.array([events[k][0].elapsed_time(events[k][n_st]) for k in range(steps)])
-    stage_ms = {name: float(np.mean([events[k][j].elapsed_time(events[k][j + 1])
-                                     for k in range(steps)]))
+    # per-stage medians: one stalled frame (a host hiccup between launches)
+    # would otherwise dominate a mean over the pass
+    stage_ms = {name: float(np.median([events[k][j].elapsed_time(events[k][j + 1])
+                                       for k in range(steps)]))
                 for j, (name, _) in enumerate(stages)}
     return wall, frame_ms, stage_ms
 
