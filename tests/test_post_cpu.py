@@ -330,3 +330,22 @@ def test_process_matches_float64_formulation(post, size, changes, ev, bloom, ton
     got = o.process(hdr, ev, bloom=bloom, tone_map=tone_map)
     want = numpy_process(hdr, o.config, ev, bloom, tone_map)
     np.testing.assert_allclose(got, want, rtol=2e-5, atol=2e-6)
+
+
+def test_abi_error_paths_without_gpu_work(pkg):
+    """Status codes of the new entry points for arguments rejected before
+    any device call: a NULL post-process handle, NULL shading fields."""
+    capi = __import__(pkg.__name__ + "._capi", fromlist=["_capi"])
+    lib = capi.library()
+    assert lib.atmo_post_process(None, None, None, 1.0, 1, 1, None) == capi.ATMO_EHANDLE
+    assert lib.atmo_post_destroy(None) == capi.ATMO_EHANDLE
+    assert lib.atmo_post_bind_buffer(None, 0, None, 0) == capi.ATMO_EHANDLE
+    assert lib.atmo_post_last_error(None) is not None
+    fields = capi.AtmoShadeFields()  # every pointer NULL
+    params = capi.AtmoShadeParams()
+    assert lib.atmo_shade_surface(ctypes.byref(fields), ctypes.byref(params), 4, 4, None) == \
+        capi.ATMO_EINVAL
+    assert lib.atmo_shade_surface(None, ctypes.byref(params), 4, 4, None) == capi.ATMO_EINVAL
+    fake = capi.AtmoShadeFields(**{name: 16 for name in capi.SHADE_FIELD_NAMES})
+    assert lib.atmo_shade_surface(ctypes.byref(fake), ctypes.byref(params), 0, 4, None) == \
+        capi.ATMO_EINVAL
