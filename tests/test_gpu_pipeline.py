@@ -26,7 +26,7 @@ def device():
     return torch.device("cuda:0")
 
 
-def oracle_chain(scene, inputs, shading, config_post, exposure_ev):
+def oracle_chain(scene, inputs, shading, config_post, exposure_ev, view=0):
     from oracle.post_cpu import PostOracle
     from oracle.ref_cpu import RefCpu
     from oracle.surface_cpu import shade
@@ -47,16 +47,21 @@ def oracle_chain(scene, inputs, shading, config_post, exposure_ev):
                            np.asarray(inputs.sun_local, np.float64),
                            tuple(float(v) for v in light.solar_irradiance),
                            light.sun_disk_radiance, math.radians(light.sun_angular_radius_degrees),
-                           right, up, forward, inputs.tangent_half_fov, c0, c1, 0,
+                           right, up, forward, inputs.tangent_half_fov, c0, c1, int(view),
                            first["transmittance"], first["multi_scattering"], first["sky_view"],
                            first["aerial"])
+    from planet_real_time_render_amd import AtmosphereDiagnosticView
+
+    v = AtmosphereDiagnosticView(int(view))
     display = PostOracle(inputs.width, inputs.height, config_post).process(
-        hdr, exposure_ev, bloom=True, tone_map=True)
+        hdr, exposure_ev, bloom=v.uses_bloom, tone_map=v.is_radiance)
     return surface_hdr, hdr, display
 
 
-@pytest.mark.parametrize("size, seed", [((320, 180), 0), ((1920, 1080), 1)])
-def test_render_frame_matches_the_oracle_chain(pkg, scene, device, size, seed):
+@pytest.mark.parametrize("size, seed, view", [((320, 180), 0, 0), ((1920, 1080), 1, 0),
+                                              ((320, 180), 2, 1), ((320, 180), 3, 2),
+                                              ((320, 180), 4, 7)])
+def test_render_frame_matches_the_oracle_chain(pkg, scene, device, size, seed, view):
     import torch
 
     w, h = size
@@ -65,7 +70,7 @@ def test_render_frame_matches_the_oracle_chain(pkg, scene, device, size, seed):
     # the G-buffer the shading inputs hold (ocean pixels re-positioned on the water plane)
     inputs.position_view = shading.fields["position"]
     inputs.surface_id = shading.fields["surface_id"]
-    print(f"render_frame {size} seed {seed}", flush=True)
+    print(f"render_frame {size} seed {seed} view {view}", flush=True)
     atmosphere = pkg.AtmosphereRenderer(w, h, inputs.config, device)
     config_post = pkg.PostprocessConfig(exposure_ev=0.5)
     post = pkg.PostProcessor(w, h, config_post, device)
@@ -83,14 +88,15 @@ def test_render_frame_matches_the_oracle_chain(pkg, scene, device, size, seed):
                      planet_radius_m=inputs.planet_radius_m, camera_radius_m=inputs.camera_radius_m,
                      sun_local=inputs.sun_local, lighting=inputs.lighting,
                      view_basis=inputs.view_basis, tangent_half_fov=inputs.tangent_half_fov,
-                     star_contrast=inputs.star_contrast, exposure_ev=config_post.exposure_ev)
+                     star_contrast=inputs.star_contrast, exposure_ev=config_post.exposure_ev,
+                     diagnostic_view=pkg.AtmosphereDiagnosticView(view))
     torch.cuda.synchronize(device)
     got_surface = fields.surface_hdr.to_numpy()
     got_hdr, got_display = hdr.cpu().numpy(), display.cpu().numpy()
     atmosphere.close()
     post.close()
     want_surface, want_hdr, want_display = oracle_chain(scene, inputs, shading, config_post,
-                                                        config_post.exposure_ev)
+                                                        config_post.exposure_ev, view)
 
     def rel(got, want):
         floor = 1.0e-6 * np.abs(want).reshape(-1, 3).max(axis=0)
