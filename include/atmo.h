@@ -3,11 +3,16 @@
  *
  * The library replaces the six Taichi kernels of the reference's
  * `AtmosphereRenderer` (src/planet_renderer/atmosphere/renderer.py) with
- * hand-written HIP kernels for gfx950.  Each entry point below names the
+ * hand-written HIP kernels for gfx950, and with them the two steps either
+ * side of the composite: the surface shading that consumes prepare_frame's
+ * lighting (`PlanetRenderer._shade_surface`, src/planet_renderer/
+ * renderer.py:1277-1575) and the display post-process (`PostProcessor`,
+ * src/planet_renderer/postprocess.py).  Each entry point below names the
  * reference kernel it replaces.  Host policy (LUT cache keys, freeze state,
- * float64 -> float32 argument rounding) stays in the caller, exactly as the
- * reference keeps it in the Python methods `update`, `prepare_frame` and
- * `composite` (renderer.py:2057-2238).
+ * float64 -> float32 argument rounding, the exposure and bloom switches)
+ * stays in the caller, exactly as the reference keeps it in the Python
+ * methods `update`, `prepare_frame`, `composite` (renderer.py:2057-2238)
+ * and `PostProcessor.process` (postprocess.py:176-207).
  *
  * Conventions shared by every entry point:
  *  - All device pointers are plain device addresses (hipMalloc'd memory or
