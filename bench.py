@@ -426,6 +426,34 @@ def bench_full_frame(pkg, scene, device, steps: int, warmup: int, stream):
             "stage_us": {k: round(v * 1e3, 2) for k, v in stage_ms.items()}}
 
 
+def bench_surrogate_4k(pkg, scene, sur_mod, weights, device, steps: int, warmup: int, stream):
+    """Surrogate inference at 3840x2160 on a full-frame-configuration
+    G-buffer: the plain composite and the surrogate composite, each timed
+    alone after one cold LUT build."""
+    import torch
+
+    inputs = scene.make_frame(width=3840, height=2160, config_index=1)
+    frame = PlainFrame(pkg, inputs, device)
+    stages = frame.stages()
+    for _, fn in stages:
+        fn()
+    net = sur_mod.SurrogateMLP.load(weights, max_batch=1024, device=device)
+    right, up, forward = frame.basis
+    c0, c1 = inputs.star_contrast
+    plain = [stages[-1]]
+    surrogate = [("composite_surrogate", lambda: frame.r._composite_surrogate(
+        net, frame.scene, frame.space, frame.sid, frame.pos, frame.hdr, frame.R, frame.cam_alt,
+        frame.alt_key, frame.sun, frame.E, frame.disk, frame.ar, right, up, forward,
+        inputs.tangent_half_fov, c0, c1))]
+    p_wall, _, _ = time_frames(plain, steps, warmup, stream, device)
+    s_wall, _, _ = time_frames(surrogate, steps, warmup, stream, device)
+    net.close()
+    frame.r.close()
+    return {"workload": "3840x2160 G-buffer of the full-frame configuration, LUTs built once",
+            "plain_composite_ms": round(p_wall, 4), "surrogate_composite_ms": round(s_wall, 4),
+            "speedup": round(p_wall / s_wall, 3)}
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -526,6 +554,12 @@ def main() -> int:
            if "cpu_oracle_ms" in shading else ""))
 
     full_frame = bench_full_frame(pkg, scene, device, args.steps, args.warmup, stream)
+    surrogate_4k = None
+    if surrogate is not None:
+        surrogate_4k = bench_surrogate_4k(pkg, scene, sur_mod, weights, device,
+                                          max(args.steps // 10, 5), 3, stream)
+        log(f"4K composite: plain {surrogate_4k['plain_composite_ms']:.4f} ms, surrogate "
+            f"{surrogate_4k['surrogate_composite_ms']:.4f} ms")
     log(f"full atmosphere frame with shading and post-process: {full_frame['wall_ms']:.4f} "
         "ms/frame (wall)")
 
@@ -683,6 +717,7 @@ def main() -> int:
         "postprocess": postprocess,
         "shading": shading,
         "full_frame": full_frame,
+        "surrogate_4k": surrogate_4k,
     }
     frame.r.close()
     print(json.dumps(result), flush=True)
