@@ -13,18 +13,20 @@
 //    ping-pong between two LDS copies (100 KB at P = 4, 156 KB at P = 8),
 //    each pass over the shrinking region the next one reads.  A lane
 //    produces 4 consecutive outputs along the pass axis from a sliding
-//    window of 8.  Tiles whose halo lies inside the frame (180 of 240 at
-//    1080p) run with compile-time regions and no clamping; the others
-//    clamp every window position as the reference clamps its taps.  The
-//    tile's interior goes to bloom_a (last vertical pass) and bloom_b (last
-//    horizontal pass), the fields the reference leaves;
+//    window of 8.  Every pass runs over compile-time regions with no
+//    clamping: a tile that reaches past the frame (60 of 240 at 1080p)
+//    fills its cells outside the frame with the values at their clamped
+//    positions, after the extraction and after each pass, so its taps
+//    read what the reference's clamped taps read.  The tile's interior
+//    goes to bloom_a (last vertical pass) and bloom_b (last horizontal
+//    pass), the fields the reference leaves;
 //  - k_post_resolve_bloom: four consecutive pixels per lane, 16-byte loads
 //    and stores, the bloom taps of all four pixels loaded before any
 //    arithmetic; without bloom every float of the frame is independent and
 //    k_post_resolve_flat streams the frame as float4s.
 // Every texel goes through the reference's operations in the reference's
 // order, so the result does not depend on the tiling: each pass reads the
-// same clamped neighbours the full-frame pass would.
+// values of the same clamped neighbours the full-frame pass would.
 //
 // Blur tiles and bloom-resolve chunks are dealt to the 8 XCDs in
 // contiguous runs (block b runs on XCD b mod 8), so neighbouring tiles,
@@ -68,58 +70,12 @@ __device__ __forceinline__ float output_channel(float value, int32_t tone_map) {
     return tone_map ? post::display_channel(value) : fminf(fmaxf(value, 0.0f), 1.0f);
 }
 
-// One blur pass over the local region [lo0, hi0) x [lo1, hi1) of the tile,
-// src -> dst, along axis 0 (x, _blur_horizontal) or axis 1 (y,
-// _blur_vertical) (postprocess.py:121-139).  A lane produces S consecutive
-// outputs along the pass axis from a window of S + 4 texels; window
-// positions are clamped to the frame as the reference clamps each tap, then
-// into the tile so no access leaves LDS.
-template <int AXIS, int L, int LP, int N, int S>
-__device__ __forceinline__ void blur_pass(const float* __restrict__ src, float* __restrict__ dst,
-                                          int lo0, int hi0, int lo1, int hi1, int g0, int extent) {
-    // axis 0: segments along x, lanes along y; axis 1: segments along y, lanes along x
-    const int lane_lo = AXIS == 0 ? lo1 : lo0, lane_n = AXIS == 0 ? hi1 - lo1 : hi0 - lo0;
-    const int seg_lo = AXIS == 0 ? lo0 : lo1, seg_hi = AXIS == 0 ? hi0 : hi1;
-    const int nseg = (seg_hi - seg_lo + S - 1) / S;
-    if (lane_n <= 0) return;
-    const float inv_lane_n = 1.0f / (float)lane_n;  // item / lane_n below 2^12: exact
-    for (int item = (int)threadIdx.x; item < lane_n * nseg; item += kBlurThreads) {
-        const int seg = (int)(((float)item + 0.5f) * inv_lane_n);
-        const int across = lane_lo + item - seg * lane_n;
-        const int start = seg_lo + seg * S;
-        const int count = min(S, seg_hi - start);
-        int idx[S + 4];
-#pragma unroll
-        for (int k = 0; k < S + 4; ++k) {
-            const int gpos = min(max(g0 + start - 2 + k, 0), extent - 1);
-            const int l = min(max(gpos - g0, 0), L - 1);
-            idx[k] = AXIS == 0 ? l * LP + across : across * LP + l;
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            float w[S + 4];
-#pragma unroll
-            for (int k = 0; k < S + 4; ++k) w[k] = src[c * N + idx[k]];
-#pragma unroll
-            for (int j = 0; j < S; ++j) {
-                if (j < count) {
-                    float color = w[j + 2] * post::kW0;
-                    color += w[j + 1] * post::kW1;
-                    color += w[j + 3] * post::kW1;
-                    color += w[j] * post::kW2;
-                    color += w[j + 4] * post::kW2;
-                    dst[c * N + idx[j + 2]] = color;
-                }
-            }
-        }
-    }
-}
-
-// The same pass on a tile whose halo lies inside the frame: no tap is
-// clamped and every region is known at compile time (pass k: x in
-// [2k, L-2k); the horizontal pass keeps y in [2k-2, L-2k+2), the vertical
-// y in [2k, L-2k)), all multiples of S = 4 long, so a lane's window reads
-// are fixed offsets from one LDS address.
+// One blur pass, src -> dst, along axis 0 (x, _blur_horizontal) or axis 1
+// (y, _blur_vertical) (postprocess.py:121-139), over a region known at
+// compile time whose length along the pass axis is a multiple of S.  A
+// lane produces S consecutive outputs from a window of S + 4 texels, read
+// at fixed offsets from one LDS address; no tap is clamped (cells outside
+// the frame already hold their clamped values).
 template <int AXIS, int L, int LP, int N, int S, int ACROSS_LO, int ACROSS_N, int SEG_LO, int NSEG>
 __device__ __forceinline__ void blur_interior(const float* __restrict__ src,
                                               float* __restrict__ dst) {
@@ -150,15 +106,52 @@ __device__ __forceinline__ void blur_interior(const float* __restrict__ src,
     }
 }
 
+// The tile's cells inside the frame: local x in [xlo, xhi), y in [ylo, yhi).
+struct Frame {
+    int xlo, xhi, ylo, yhi;
+    bool edge;
+};
+
+// Copies onto each cell of [lo0, hi0) x [lo1, hi1) outside the frame the
+// value of its clamped position, which lies inside the region whenever a
+// cell outside the frame does.  The pass over a padded tile then reads
+// what the reference's clamped taps read.
+template <int LP, int N>
+__device__ __forceinline__ void replicate(float* buf, const Frame& f, int lo0, int hi0, int lo1,
+                                          int hi1) {
+    const int n1 = hi1 - lo1, n = (hi0 - lo0) * n1;
+    for (int item = (int)threadIdx.x; item < n; item += kBlurThreads) {
+        const int x = lo0 + item / n1, y = lo1 + item % n1;
+        const int cx = min(max(x, f.xlo), f.xhi - 1), cy = min(max(y, f.ylo), f.yhi - 1);
+        if (cx == x && cy == y) continue;
+        const int d = x * LP + y, s = cx * LP + cy;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) buf[c * N + d] = buf[c * N + s];
+    }
+}
+
+// Pass pairs K..P over compile-time regions (pass k: x in [2k, L-2k); the
+// horizontal pass keeps y in [2k-2, L-2k+2), the vertical y in
+// [2k, L-2k)).  On a tile that reaches past the frame every pass is
+// followed by the copy of clamped values onto the cells outside it (the
+// last vertical pass's are never read).
 template <int L, int LP, int N, int K, int P>
-__device__ __forceinline__ void interior_passes(float* A, float* B) {
+__device__ __forceinline__ void blur_passes(float* A, float* B, const Frame& f) {
     if constexpr (K <= P) {
         constexpr int S = 4, NSEG = (L - 4 * K) / 4;
         blur_interior<0, L, LP, N, S, 2 * K - 2, L - 4 * K + 4, 2 * K, NSEG>(A, B);
         __syncthreads();
+        if (f.edge) {
+            replicate<LP, N>(B, f, 2 * K, L - 2 * K, 2 * K - 2, L - 2 * K + 2);
+            __syncthreads();
+        }
         blur_interior<1, L, LP, N, S, 2 * K, L - 4 * K, 2 * K, NSEG>(B, A);
         __syncthreads();
-        interior_passes<L, LP, N, K + 1, P>(A, B);
+        if (K < P && f.edge) {
+            replicate<LP, N>(A, f, 2 * K, L - 2 * K, 2 * K, L - 2 * K);
+            __syncthreads();
+        }
+        blur_passes<L, LP, N, K + 1, P>(A, B, f);
     }
 }
 
@@ -171,7 +164,6 @@ __global__ __launch_bounds__(kBlurThreads) void k_post_blur(PostArgs a) {
     constexpr int L = kTile + 2 * R;
     constexpr int LP = L | 1;             // odd pitch: column walks hit distinct banks
     constexpr int N = L * LP;
-    constexpr int S = 4;                  // outputs per lane along the pass axis
     __shared__ float lds[2 * 3 * N];
     float* A = lds;
     float* B = lds + 3 * N;
@@ -184,20 +176,19 @@ __global__ __launch_bounds__(kBlurThreads) void k_post_blur(PostArgs a) {
 
     // _extract (postprocess.py:110-119) over the whole local region, in
     // chunks of CH texels per lane whose hdr loads are all issued before
-    // the first arithmetic; texels outside the frame are zero and never read
+    // the first arithmetic.  A cell outside the frame holds the texel at
+    // its clamped position, the value the reference's clamped taps read.
     constexpr int NL = (L * L + kBlurThreads - 1) / kBlurThreads;
     constexpr int CH = 4;
 #pragma unroll
     for (int k0 = 0; k0 < NL; k0 += CH) {
         float px[CH][12];
-        bool in[CH];
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
             const int i = (int)threadIdx.x + (k0 + c) * kBlurThreads;
             const int lx = i / L, ly = i - lx * L;
-            const int gx = gx0 + lx, gy = gy0 + ly;
-            in[c] = k0 + c < NL && i < L * L && gx >= 0 && gx < hw && gy >= 0 && gy < hh;
-            if (in[c]) {
+            const int gx = min(max(gx0 + lx, 0), hw - 1), gy = min(max(gy0 + ly, 0), hh - 1);
+            if (k0 + c < NL && i < L * L) {
 #pragma unroll
                 for (int ox = 0; ox < 2; ++ox) {
                     const int sx = min(gx * 2 + ox, a.width - 1);
@@ -216,19 +207,17 @@ __global__ __launch_bounds__(kBlurThreads) void k_post_blur(PostArgs a) {
             const int i = (int)threadIdx.x + (k0 + c) * kBlurThreads;
             if (k0 + c >= NL || i >= L * L) continue;
             float r = 0.0f, g = 0.0f, b = 0.0f;
-            if (in[c]) {
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    r += px[c][3 * t];
-                    g += px[c][3 * t + 1];
-                    b += px[c][3 * t + 2];
-                }
-                const float s = 0.25f * a.exposure_multiplier;
-                r *= s;
-                g *= s;
-                b *= s;
-                post::soft_threshold(a.bright, r, g, b);
+            for (int t = 0; t < 4; ++t) {
+                r += px[c][3 * t];
+                g += px[c][3 * t + 1];
+                b += px[c][3 * t + 2];
             }
+            const float s = 0.25f * a.exposure_multiplier;
+            r *= s;
+            g *= s;
+            b *= s;
+            post::soft_threshold(a.bright, r, g, b);
             const int lx = i / L, ly = i - lx * L;
             const int li = lx * LP + ly;
             A[li] = r;
@@ -237,23 +226,13 @@ __global__ __launch_bounds__(kBlurThreads) void k_post_blur(PostArgs a) {
         }
     }
     __syncthreads();
-    if (gx0 >= 0 && gx0 + L <= hw && gy0 >= 0 && gy0 + L <= hh) {
-        interior_passes<L, LP, N, 1, P>(A, B);
-    } else {
-#pragma unroll 1
-        for (int pass = 1; pass <= P; ++pass) {
-            const int xlo = max(gx0 + 2 * pass, 0) - gx0;
-            const int xhi = min(X0 + kTile + R - 2 * pass, hw) - gx0;
-            const int ylo_h = max(gy0 + 2 * (pass - 1), 0) - gy0;
-            const int yhi_h = min(Y0 + kTile + R - 2 * (pass - 1), hh) - gy0;
-            blur_pass<0, L, LP, N, S>(A, B, xlo, xhi, ylo_h, yhi_h, gx0, hw);
-            __syncthreads();
-            const int ylo = max(gy0 + 2 * pass, 0) - gy0;
-            const int yhi = min(Y0 + kTile + R - 2 * pass, hh) - gy0;
-            blur_pass<1, L, LP, N, S>(B, A, xlo, xhi, ylo, yhi, gy0, hh);
-            __syncthreads();
-        }
-    }
+    Frame f;
+    f.xlo = max(-gx0, 0);
+    f.xhi = min(hw - gx0, L);
+    f.ylo = max(-gy0, 0);
+    f.yhi = min(hh - gy0, L);
+    f.edge = f.xlo > 0 || f.xhi < L || f.ylo > 0 || f.yhi < L;
+    blur_passes<L, LP, N, 1, P>(A, B, f);
     for (int i = threadIdx.x; i < kTile * kTile; i += kBlurThreads) {
         const int tx = i / kTile, ty = i - tx * kTile;
         const int gx = X0 + tx, gy = Y0 + ty;
