@@ -35,10 +35,11 @@ __device__ __noinline__ float srgb_pow_slow(float m) {
 // with 1/y0 from the float reciprocal and two float64 Newton steps, so z
 // carries ~2^-52.  Then m^p = z * m^-delta, delta = 5/12 - p = 2^-25/3,
 // through exp(-delta ln m) = 1 + g + g^2/2 with g = -delta ln m; ln m comes
-// from the exponent of m and the hardware log2 of its mantissa in
-// [0.5, 1), good to ~2^-23 absolute, which puts ~2^-50 into the result.
-// If the float64 value lies within 2^-47 of a float32 rounding boundary,
-// the library power decides (about one input in four million).
+// from the seed's hardware log2 of m.  |log2 m| <= 8.33 on the domain, so
+// a log2 good to 1 ulp (2^-20 there) puts at most delta ln2 2^-20 = 2^-47.1
+// into the result, and the other terms ~2^-50.  If the float64 value lies
+// within 2^-45 of a float32 rounding boundary, twice that budget even for
+// a 2-ulp log2, the library power decides (about one input in 1.4 million).
 __device__ __forceinline__ float srgb_pow(float m) {
     const float l2 = __builtin_amdgcn_logf(m);
     const float y0f = __builtin_amdgcn_exp2f(kSrgbExponent * l2);
@@ -51,13 +52,11 @@ __device__ __forceinline__ float srgb_pow(float m) {
     const double w2 = w * w, w4 = w2 * w2, w8 = w4 * w4, w12 = w8 * w4;
     const double q = fma(m5, w12, -1.0);
     const double t = q * fma(q, -11.0 / 288.0, 1.0 / 12.0);
-    const double lg = (double)__builtin_amdgcn_frexp_expf(m) +
-                      (double)__builtin_amdgcn_logf(__builtin_amdgcn_frexp_mantf(m));
-    const double g = lg * -kDeltaLn2;
+    const double g = (double)l2 * -kDeltaLn2;
     const double c = fma(g, fma(g, 0.5, 1.0), 1.0);
     const double y = fma(y0, t, y0) * c;
-    const float up = (float)(y * (1.0 + 0x1p-47));
-    const float down = (float)(y * (1.0 - 0x1p-47));
+    const float up = (float)(y * (1.0 + 0x1p-45));
+    const float down = (float)(y * (1.0 - 0x1p-45));
     return up == down ? up : srgb_pow_slow(m);
 }
 
