@@ -82,6 +82,8 @@ CASES = [
     ((33, 1), {}, 0.0, True, True),
     ((1, 1), {}, 0.0, True, True),
     ((2, 3), {"bloom_passes": 2}, 0.0, True, True),
+    ((5, 3), {"bloom_passes": 8}, 1.0, True, True),
+    ((3, 201), {"bloom_passes": 8}, 0.0, True, True),
     ((300, 170), {}, 2.0, False, True),
     ((300, 170), {}, 0.0, False, False),
     ((300, 170), {"bloom_passes": 0}, 0.0, True, True),
