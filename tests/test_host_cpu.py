@@ -133,6 +133,24 @@ def test_create_rejects_invalid_arguments_without_gpu_work(pkg):
     assert lib.atmo_build_transmittance(None, 6.36e6, None) == capi.ATMO_EHANDLE
 
 
+def test_missing_library_and_device_raise(pkg, monkeypatch, tmp_path):
+    """No CPU path: without the built library, or without a HIP device, the
+    entry points raise instead of computing anything elsewhere."""
+    capi = __import__(pkg.__name__ + "._capi", fromlist=["_capi"])
+    monkeypatch.setattr(capi, "LIBRARY_PATH", tmp_path / "libatmo.so")
+    monkeypatch.setattr(capi, "_LIB", None)
+    with pytest.raises(RuntimeError, match="is missing"):
+        capi.library()
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is present")
+    with pytest.raises(RuntimeError, match="HIP device"):
+        pkg.PostProcessor(8, 6, pkg.PostprocessConfig())
+    with pytest.raises(RuntimeError):
+        pkg.AtmosphereRenderer(8, 6, pkg.AtmosphereConfig())
+
+
 # ------------------------------------------------------- LUT cache policy
 
 class _PolicyProbe:
