@@ -12,14 +12,15 @@
 //    lane's hdr loads issued before its arithmetic), then runs the passes
 //    ping-pong between two LDS copies (100 KB at P = 4, 156 KB at P = 8),
 //    each pass over the shrinking region the next one reads.  A lane
-//    produces 4 consecutive outputs along the pass axis from a sliding
-//    window of 8.  Every pass runs over compile-time regions with no
-//    clamping: a tile that reaches past the frame (60 of 240 at 1080p)
-//    fills its cells outside the frame with the values at their clamped
-//    positions, after the extraction and after each pass, so its taps
-//    read what the reference's clamped taps read.  The tile's interior
-//    goes to bloom_a (last vertical pass) and bloom_b (last horizontal
-//    pass), the fields the reference leaves;
+//    produces S consecutive outputs of one channel along the pass axis
+//    from a window of S + 4, S chosen per pass (8 to 19) so that every
+//    lane has at most one window.  Every pass runs over compile-time
+//    regions with no clamping: a tile that reaches past the frame (60 of
+//    240 at 1080p) fills its cells outside the frame with the values at
+//    their clamped positions, after the extraction and after each pass,
+//    so its taps read what the reference's clamped taps read.  The tile's
+//    interior goes to bloom_a (last vertical pass) and bloom_b (last
+//    horizontal pass), the fields the reference leaves;
 //  - k_post_resolve_bloom: four consecutive pixels per lane, 16-byte loads
 //    and stores, the bloom taps of all four pixels loaded before any
 //    arithmetic; without bloom every float of the frame is independent and
@@ -80,30 +81,42 @@ template <int AXIS, int L, int LP, int N, int S, int ACROSS_LO, int ACROSS_N, in
 __device__ __forceinline__ void blur_interior(const float* __restrict__ src,
                                               float* __restrict__ dst) {
     constexpr int STEP = AXIS == 0 ? LP : 1, ASTEP = AXIS == 0 ? 1 : LP;
-    for (int item = (int)threadIdx.x; item < NSEG * ACROSS_N; item += kBlurThreads) {
-        const int seg = item / ACROSS_N;
-        const int across = ACROSS_LO + item - seg * ACROSS_N;
+    constexpr int PER_CHANNEL = NSEG * ACROSS_N;
+    for (int item = (int)threadIdx.x; item < 3 * PER_CHANNEL; item += kBlurThreads) {
+        const int c = item / PER_CHANNEL;
+        const int rest = item - c * PER_CHANNEL;
+        const int seg = rest / ACROSS_N;
+        const int across = ACROSS_LO + rest - seg * ACROSS_N;
         const int start = SEG_LO + seg * S;
         // element (position p along the pass axis, across) at p * STEP + across * ASTEP
-        const int base = (start - 2) * STEP + across * ASTEP;
+        const int base = c * N + (start - 2) * STEP + across * ASTEP;
+        const float* s = src + base;
+        float* d = dst + base;
+        float w[S + 4];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float* s = src + c * N + base;
-            float* d = dst + c * N + base;
-            float w[S + 4];
+        for (int k = 0; k < S + 4; ++k) w[k] = s[k * STEP];
 #pragma unroll
-            for (int k = 0; k < S + 4; ++k) w[k] = s[k * STEP];
-#pragma unroll
-            for (int j = 0; j < S; ++j) {
-                float color = w[j + 2] * post::kW0;
-                color += w[j + 1] * post::kW1;
-                color += w[j + 3] * post::kW1;
-                color += w[j] * post::kW2;
-                color += w[j + 4] * post::kW2;
-                d[(j + 2) * STEP] = color;
-            }
+        for (int j = 0; j < S; ++j) {
+            float color = w[j + 2] * post::kW0;
+            color += w[j + 1] * post::kW1;
+            color += w[j + 3] * post::kW1;
+            color += w[j] * post::kW2;
+            color += w[j + 4] * post::kW2;
+            d[(j + 2) * STEP] = color;
         }
     }
+}
+
+// Outputs per lane for a pass region `len` texels long and `across` wide:
+// the smallest divisor of len (a multiple of 4) that gives every lane at
+// most one item, so the pass is one short chain per lane; the largest
+// divisor up to 20 if none does.
+constexpr int window_outputs(int len, int across) {
+    for (int s = 4; s <= len; ++s)
+        if (len % s == 0 && 3 * (len / s) * across <= kBlurThreads) return s;
+    for (int s = 20; s > 4; --s)
+        if (len % s == 0) return s;
+    return 4;
 }
 
 // The tile's cells inside the frame: local x in [xlo, xhi), y in [ylo, yhi).
@@ -138,14 +151,15 @@ __device__ __forceinline__ void replicate(float* buf, const Frame& f, int lo0, i
 template <int L, int LP, int N, int K, int P>
 __device__ __forceinline__ void blur_passes(float* A, float* B, const Frame& f) {
     if constexpr (K <= P) {
-        constexpr int S = 4, NSEG = (L - 4 * K) / 4;
-        blur_interior<0, L, LP, N, S, 2 * K - 2, L - 4 * K + 4, 2 * K, NSEG>(A, B);
+        constexpr int SH = window_outputs(L - 4 * K, L - 4 * K + 4);
+        constexpr int SV = window_outputs(L - 4 * K, L - 4 * K);
+        blur_interior<0, L, LP, N, SH, 2 * K - 2, L - 4 * K + 4, 2 * K, (L - 4 * K) / SH>(A, B);
         __syncthreads();
         if (f.edge) {
             replicate<LP, N>(B, f, 2 * K, L - 2 * K, 2 * K - 2, L - 2 * K + 2);
             __syncthreads();
         }
-        blur_interior<1, L, LP, N, S, 2 * K, L - 4 * K, 2 * K, NSEG>(B, A);
+        blur_interior<1, L, LP, N, SV, 2 * K, L - 4 * K, 2 * K, (L - 4 * K) / SV>(B, A);
         __syncthreads();
         if (K < P && f.edge) {
             replicate<LP, N>(A, f, 2 * K, L - 2 * K, 2 * K, L - 2 * K);
