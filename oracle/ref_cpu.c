@@ -1615,3 +1615,24 @@ EXPORT void oracle_fn_solar_shadow_roots(const real origin[3], const real ray[3]
     out[2] = (real)v0;
     out[3] = (real)v1;
 }
+
+/* renderer.py:1788-1825 for one ray (camera at altitude h on the local
+ * vertical axis). */
+EXPORT void oracle_fn_camera_transmittance(const AtmoParams* p, const real* tlut, real h,
+                                           const real ray[3], real bottom, real out[3]) {
+    K k = make_consts(p);
+    v3 t = camera_transmittance(&k, tlut, h, varr(ray), bottom);
+    out[0] = t.x; out[1] = t.y; out[2] = t.z;
+}
+
+/* renderer.py:901-966 for one base interval [start, end] with the given
+ * shadow roots and their validity flags. */
+EXPORT real oracle_fn_terminator_weight(const AtmoParams* p, const real origin[3], real h0,
+                                        const real ray[3], real start, real end,
+                                        const real sun[3], real bottom, real angular_radius,
+                                        const real roots[2], const int32_t valid[2]) {
+    K k = make_consts(p);
+    return terminator_refinement_weight(&k, varr(origin), h0, varr(ray), start, end,
+                                        varr(sun), bottom, angular_radius, roots[0],
+                                        roots[1], valid[0], valid[1]);
+}

@@ -83,8 +83,10 @@ def build(force: bool = False) -> None:
 
 
 class RefCpu:
-    def __init__(self, precision: str = "f32"):
-        path = LIBS[precision]
+    def __init__(self, precision: str = "f32", path: Path | None = None):
+        """`path` loads another build of the same source with the given
+        precision (the tests' deliberately broken variants)."""
+        path = LIBS[precision] if path is None else Path(path)
         if not path.exists():
             raise RuntimeError(f"{path} missing: run `make -C oracle`")
         self.precision = precision
