@@ -1261,9 +1261,12 @@ __device__ __forceinline__ int compact_block(bool flag, int* list, int* wave_cou
 
 // Appends the flagged lanes' pixel indices to the deferred-pixel queue (one
 // atomic per wave, by the lowest flagged lane).  Every lane of the wave
-// must call it.
+// must call it.  `capacity` is the number of index slots after the two
+// counters; an index past it is dropped (a counter that did not start at
+// zero can only come from an earlier call that failed to launch its list
+// pass, and the host clears the counters before the next call then).
 __device__ __forceinline__ void enqueue_wave(bool flag, long pixel, int* __restrict__ queue,
-                                             int parity) {
+                                             int parity, long capacity) {
     const unsigned long long mask = __ballot(flag);
     if (!mask) return;
     const int lane = threadIdx.x % 64;
@@ -1271,7 +1274,8 @@ __device__ __forceinline__ void enqueue_wave(bool flag, long pixel, int* __restr
     int slot = 0;
     if (lane == leader) slot = atomicAdd(queue + parity, __popcll(mask));
     slot = __shfl(slot, leader, 64);
-    if (flag) queue[2 + slot + __popcll(mask & ((1ull << lane) - 1ull))] = (int)pixel;
+    const long at = (long)slot + __popcll(mask & ((1ull << lane) - 1ull));
+    if (flag && at < capacity) queue[2 + at] = (int)pixel;
 }
 
 }  // inline namespace ATMO_PRECISION

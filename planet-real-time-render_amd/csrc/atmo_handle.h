@@ -30,6 +30,7 @@ struct AtmoHandle {
     // counts in [parity]; its last pass zeroes [1 - parity] for the next.
     int* queue;
     int queue_parity;
+    bool queue_reset;  // a composite launch failed: clear both counters first
     int precision;     // ATMO_PRECISION_REFERENCE or ATMO_PRECISION_ROBUST
     int group_t;       // lanes per transmittance texel (4, 8, 16), autotuned
     int group_sv;      // lanes per sky-view texel (4, 8, 16), autotuned
@@ -56,6 +57,7 @@ struct CompositeArgs {
     int view;
     int* queue;  // COMPOSITE view: deferred pixels, layout of AtmoHandle::queue
     int parity;  // the counter of this call: queue[parity]
+    long queue_capacity;  // index slots after the two counters
     // frame constants (composite_frame_constants)
     float aspect;        // width / height
     atmo::f3 fwd_t;      // forward / thf
@@ -90,6 +92,15 @@ struct BatchState {
 // (atmo_kernels.hip).
 int32_t atmo_launch_composite_sky(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream);
 int32_t atmo_launch_composite_list(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream);
+
+// Clears both counters of a deferred-pixel queue with a one-wave kernel
+// (no memset node, so the launch can sit in a captured graph).
+void atmo_launch_clear_queue(int* queue, hipStream_t stream);
+
+// Prepares the handle's queue for one COMPOSITE-view call: clears the
+// counters if an earlier call failed, and fills a.queue, a.parity and
+// a.queue_capacity (atmo_kernels.hip).
+int32_t atmo_begin_queue(AtmoHandle* h, CompositeArgs& a, hipStream_t stream);
 
 inline atmo::f3 mk3(const float* a) { return atmo::f3{a[0], a[1], a[2]}; }
 

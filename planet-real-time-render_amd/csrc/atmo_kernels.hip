@@ -257,6 +257,8 @@ int32_t atmo_create(const AtmoParams* params, int32_t width, int32_t height, Atm
         h->borrowed[b] = false;
     }
     h->queue = nullptr;
+    h->queue_parity = 0;
+    h->queue_reset = false;
     if (hipGetDevice(&h->device) != hipSuccess) {
         set_error(nullptr, "atmo_create: no HIP device");
         delete h;
@@ -443,6 +445,33 @@ int32_t atmo_launch_composite_list(AtmoHandle* h, const CompositeArgs& a, hipStr
     return atmo_launch_status(h, "k_composite_list");
 }
 
+namespace {
+
+__global__ void k_clear_queue(int* queue) {
+    if (threadIdx.x < 2) queue[threadIdx.x] = 0;
+}
+
+}  // namespace
+
+void atmo_launch_clear_queue(int* queue, hipStream_t stream) {
+    hipLaunchKernelGGL(k_clear_queue, dim3(1), dim3(64), 0, stream, queue);
+}
+
+int32_t atmo_begin_queue(AtmoHandle* h, CompositeArgs& a, hipStream_t stream) {
+    if (h->queue_reset) {
+        atmo_launch_clear_queue(h->queue, stream);
+        if (atmo_launch_status(h, "atmo_composite: clearing the pixel queue") != ATMO_OK)
+            return ATMO_EHIP;
+        h->queue_parity = 0;
+        h->queue_reset = false;
+    }
+    a.queue = h->queue;
+    a.parity = h->queue_parity;
+    a.queue_capacity = (long)h->width * h->height;
+    h->queue_parity ^= 1;
+    return ATMO_OK;
+}
+
 extern "C" {
 
 int32_t atmo_composite(AtmoHandle* h, const float* scene_hdr, const float* space_hdr,
@@ -489,10 +518,19 @@ int32_t atmo_composite(AtmoHandle* h, const float* scene_hdr, const float* space
     a.view = view;
     a.queue = h->queue;
     a.parity = h->queue_parity;
+    a.queue_capacity = (long)h->width * h->height;
     composite_frame_constants(h->k, a);
-    if (view == ATMO_VIEW_COMPOSITE) h->queue_parity ^= 1;
+    if (view != ATMO_VIEW_COMPOSITE) {
+        ATMO_STAGE(h, launch_composite, h->k, a, (hipStream_t)stream);
+        return atmo_launch_status(h, "atmo_composite");
+    }
+    if (atmo_begin_queue(h, a, (hipStream_t)stream) != ATMO_OK) return ATMO_EHIP;
     ATMO_STAGE(h, launch_composite, h->k, a, (hipStream_t)stream);
-    return atmo_launch_status(h, "atmo_composite");
+    if (atmo_launch_status(h, "atmo_composite") != ATMO_OK) {
+        h->queue_reset = true;
+        return ATMO_EHIP;
+    }
+    return ATMO_OK;
 }
 
 }  // extern "C"

@@ -586,7 +586,7 @@ k_composite(Consts k, CompositeArgs a) {
             refine = true;
         }
     }
-    enqueue_wave(refine, j, a.queue, a.parity);
+    enqueue_wave(refine, j, a.queue, a.parity, a.queue_capacity);
 }
 
 // Sky pixels only: the sky pass of the surrogate composite.
@@ -601,7 +601,7 @@ __global__ void __launch_bounds__(256) k_composite_sky(Consts k, CompositeArgs a
 // finish, or the surrogate composite's out-of-domain pixels).  Block 0 then
 // zeroes the other counter for the next call.
 __global__ void __launch_bounds__(256) k_composite_list(Consts k, CompositeArgs a) {
-    const int count = a.queue[a.parity];
+    const int count = (int)min((long)a.queue[a.parity], a.queue_capacity);
     if (blockIdx.x == 0 && threadIdx.x == 0) a.queue[1 - a.parity] = 0;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < count; e += gridDim.x * blockDim.x) {
         const long j = a.queue[2 + e];

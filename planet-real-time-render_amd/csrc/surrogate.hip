@@ -539,7 +539,7 @@ __global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, Surrogate
         const float sd = has ? norm(ld3(a.pos, j)) : 1.0f;
         const bool learned = has && in_domain(a.t, a.cam_alt, sd);
         // queue the out-of-domain surface pixels
-        enqueue_wave(has && !learned, j, queue, parity);
+        enqueue_wave(has && !learned, j, queue, parity, npix);
         if (__ballot(learned)) {  // wave-uniform branch
             float x[5];
             // relative azimuth as in the sky-view sampler (renderer.py:1482-1497)
@@ -907,17 +907,18 @@ int32_t atmo_composite_surrogate(AtmoHandle* h, AtmoMlp* m, const float* scene_h
     ca.c0 = c0;
     ca.c1 = c1;
     ca.view = ATMO_VIEW_COMPOSITE;
-    ca.queue = h->queue;
-    ca.parity = h->queue_parity;
     composite_frame_constants(h->k, ca);
     a.aspect = ca.aspect;
     a.fwd_t = ca.fwd_t;
-    h->queue_parity ^= 1;
+    if (atmo_begin_queue(h, ca, s) != ATMO_OK) return ATMO_EHIP;
+    h->queue_reset = true;  // until every launch of this call has gone out
     if (atmo_launch_composite_sky(h, ca, s) != ATMO_OK) return ATMO_EHIP;
     hipLaunchKernelGGL(k_composite_surrogate, dim3((unsigned)(groups < 1024 ? groups : 1024)),
                        dim3(256), 0, s, h->k, a, h->queue, ca.parity);
     if (atmo_launch_status(h, "atmo_composite_surrogate") != ATMO_OK) return ATMO_EHIP;
-    return atmo_launch_composite_list(h, ca, s);
+    if (atmo_launch_composite_list(h, ca, s) != ATMO_OK) return ATMO_EHIP;
+    h->queue_reset = false;
+    return ATMO_OK;
 }
 
 }  // extern "C"
