@@ -225,6 +225,31 @@ int32_t atmo_composite(
     float tangent_half_fov, float star_contrast_start,
     float star_contrast_end, int32_t diagnostic_view, void* stream);
 
+/* No counterpart in the reference.  atmo_composite for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * hosts that forbid hipMalloc): stateless -- no handle; the caller passes
+ * the configuration, the four LUTs (the layouts of ATMO_BUF_TRANSMITTANCE ..
+ * ATMO_BUF_AERIAL_SCATTERING) and the frame size along with atmo_composite's
+ * arguments -- and the deferred-pixel queue of the COMPOSITE view lives in
+ * a library-owned static device array of ATMO_NOALLOC_MAX_PIXELS + 2 int32
+ * (32 MiB per GPU, held from the library's first use of that GPU until the
+ * process ends).  Reference precision.  Frames above ATMO_NOALLOC_MAX_PIXELS
+ * pixels return ATMO_ESHAPE; errors are reported by atmo_last_error(NULL).
+ * Calls on one GPU share that GPU's queue: calls that overlap in time on
+ * one GPU (another stream, another thread) race.  Each COMPOSITE call clears
+ * the queue's counters with a one-wave kernel first.  Results are bitwise
+ * those of atmo_composite. */
+#define ATMO_NOALLOC_MAX_PIXELS 8388608 /* 2^23: covers 3840 x 2160 */
+int32_t atmo_composite_noalloc(
+    const AtmoParams* params, const float* transmittance_lut, const float* multi_scattering_lut,
+    const float* sky_view_lut, const float* aerial_lut, int32_t width, int32_t height,
+    const float* scene_hdr, const float* space_hdr, const int32_t* surface_id,
+    const float* position_view, float* hdr, float bottom_radius, float camera_altitude,
+    float sky_lut_camera_altitude, const float sun_local[3], const float solar_irradiance[3],
+    const float sun_disk_radiance[3], float sun_angular_radius, const float right[3],
+    const float view_up[3], const float forward[3], float tangent_half_fov,
+    float star_contrast_start, float star_contrast_end, int32_t diagnostic_view, void* stream);
+
 /* ------------------------------------------------------------------------
  * Batched LUT builds.  The reference renders one atmosphere state per
  * AtmosphereRenderer; a batch builds the Transmittance, Multi-scattering,

@@ -29,6 +29,8 @@ BUF_SURFACE_SKY_L = 5
 PRECISION_REFERENCE = 0
 PRECISION_ROBUST = 1
 
+NOALLOC_MAX_PIXELS = 8388608  # ATMO_NOALLOC_MAX_PIXELS
+
 # Every symbol include/atmo.h declares, checked by the CPU test suite.
 EXPORTED_SYMBOLS = (
     "atmo_validate_params",
@@ -47,6 +49,7 @@ EXPORTED_SYMBOLS = (
     "atmo_build_aerial_perspective",
     "atmo_prepare_surface_lighting",
     "atmo_composite",
+    "atmo_composite_noalloc",
     "atmo_batch_create",
     "atmo_batch_destroy",
     "atmo_batch_last_error",
@@ -235,6 +238,11 @@ def _signatures(lib: ctypes.CDLL) -> None:
     ]
     lib.atmo_composite.argtypes = [
         _P, _P, _P, _P, _P, _P, _F, _F, _F, _PF3, _PF3, _PF3, _F, _PF3, _PF3, _PF3,
+        _F, _F, _F, _I, _P,
+    ]
+    lib.atmo_composite_noalloc.argtypes = [
+        ctypes.POINTER(AtmoParams), _P, _P, _P, _P, _I, _I,
+        _P, _P, _P, _P, _P, _F, _F, _F, _PF3, _PF3, _PF3, _F, _PF3, _PF3, _PF3,
         _F, _F, _F, _I, _P,
     ]
     _PI = ctypes.POINTER(ctypes.c_int32)

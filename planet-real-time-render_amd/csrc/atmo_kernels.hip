@@ -10,6 +10,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 
 #include "../../include/atmo.h"
@@ -528,6 +529,125 @@ int32_t atmo_composite(AtmoHandle* h, const float* scene_hdr, const float* space
     ATMO_STAGE(h, launch_composite, h->k, a, (hipStream_t)stream);
     if (atmo_launch_status(h, "atmo_composite") != ATMO_OK) {
         h->queue_reset = true;
+        return ATMO_EHIP;
+    }
+    return ATMO_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------- the allocation-free composite
+
+namespace {
+
+// The deferred-pixel queue of atmo_composite_noalloc: two counters, then
+// up to ATMO_NOALLOC_MAX_PIXELS pixel indices (32 MiB).  Every GPU the
+// library runs on holds its own copy from the load of the code object
+// until the process ends; calls on one GPU share it.
+__device__ __align__(256) int scratch[ATMO_NOALLOC_MAX_PIXELS + 2];
+
+constexpr int kMaxDevices = 64;
+int* g_scratch[kMaxDevices];  // scratch's device address, per device
+std::mutex g_scratch_mutex;
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current.  NULL with the message in g_create_error
+// on failure.
+int* scratch_on_current_device() {
+    int device = -1;
+    if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices) {
+        set_error(nullptr, "atmo_composite_noalloc: no usable HIP device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error(nullptr, "atmo_composite_noalloc: hipGetSymbolAddress(scratch): %s",
+                      hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch[device] = static_cast<int*>(ptr);
+    }
+    return g_scratch[device];
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t atmo_composite_noalloc(const AtmoParams* params, const float* transmittance_lut,
+                               const float* multi_scattering_lut, const float* sky_view_lut,
+                               const float* aerial_lut, int32_t width, int32_t height,
+                               const float* scene_hdr, const float* space_hdr,
+                               const int32_t* surface_id, const float* position_view, float* hdr,
+                               float bottom, float cam_alt, float sky_alt, const float sun[3],
+                               const float irradiance[3], const float disk[3], float ar,
+                               const float right[3], const float up[3], const float forward[3],
+                               float thf, float c0, float c1, int32_t view, void* stream) {
+    if (!params || !transmittance_lut || !multi_scattering_lut || !sky_view_lut || !aerial_lut ||
+        !scene_hdr || !space_hdr || !surface_id || !position_view || !hdr || !sun ||
+        !irradiance || !disk || !right || !up || !forward) {
+        set_error(nullptr, "atmo_composite_noalloc: NULL argument");
+        return ATMO_EINVAL;
+    }
+    if (validate(params, g_create_error, sizeof g_create_error) != ATMO_OK) return ATMO_EINVAL;
+    if (view < ATMO_VIEW_COMPOSITE || view > ATMO_VIEW_AERIAL_BLEND_WEIGHT) {
+        set_error(nullptr, "atmo_composite_noalloc: diagnostic_view must be in 0..10");
+        return ATMO_EINVAL;
+    }
+    if (width <= 0 || height <= 0) {
+        set_error(nullptr, "atmo_composite_noalloc: width and height must be positive");
+        return ATMO_EINVAL;
+    }
+    if ((long)width * (long)height > (long)ATMO_NOALLOC_MAX_PIXELS) {
+        char msg[160];
+        snprintf(msg, sizeof msg,
+                 "atmo_composite_noalloc: %d x %d pixels exceed the limit of %d pixels",
+                 (int)width, (int)height, (int)ATMO_NOALLOC_MAX_PIXELS);
+        set_error(nullptr, "%s", msg);
+        return ATMO_ESHAPE;
+    }
+    int* queue = scratch_on_current_device();
+    if (!queue) return ATMO_EHIP;
+    const Consts k = make_consts(*params);
+    CompositeArgs a;
+    a.scene = scene_hdr;
+    a.space = space_hdr;
+    a.sid = surface_id;
+    a.pos = position_view;
+    a.hdr = hdr;
+    a.tlut = transmittance_lut;
+    a.mslut = multi_scattering_lut;
+    a.svlut = sky_view_lut;
+    a.aplut = aerial_lut;
+    a.width = width;
+    a.height = height;
+    a.bottom = bottom;
+    a.cam_alt = cam_alt;
+    a.sky_alt = sky_alt;
+    a.sun = host3(sun);
+    a.irradiance = host3(irradiance);
+    a.disk = host3(disk);
+    a.ar = ar;
+    a.right = host3(right);
+    a.up = host3(up);
+    a.forward = host3(forward);
+    a.thf = thf;
+    a.c0 = c0;
+    a.c1 = c1;
+    a.view = view;
+    a.queue = queue;
+    a.parity = 0;
+    a.queue_capacity = (long)width * height;
+    composite_frame_constants(k, a);
+    const hipStream_t s = (hipStream_t)stream;
+    if (view == ATMO_VIEW_COMPOSITE) atmo_launch_clear_queue(queue, s);
+    atmo::ref::launch_composite(k, a, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(nullptr, "atmo_composite_noalloc: %s", hipGetErrorString(e));
         return ATMO_EHIP;
     }
     return ATMO_OK;

@@ -349,3 +349,30 @@ def test_abi_error_paths_without_gpu_work(pkg):
     fake = capi.AtmoShadeFields(**{name: 16 for name in capi.SHADE_FIELD_NAMES})
     assert lib.atmo_shade_surface(ctypes.byref(fake), ctypes.byref(params), 0, 4, None) == \
         capi.ATMO_EINVAL
+
+
+def test_noalloc_composite_rejects_before_device_work(pkg):
+    """atmo_composite_noalloc's argument checks run before any device call:
+    NULL pointers, an invalid configuration, a bad view, and a frame above
+    ATMO_NOALLOC_MAX_PIXELS (the static queue's capacity)."""
+    capi = __import__(pkg.__name__ + "._capi", fromlist=["_capi"])
+    lib = capi.library()
+    params = pkg.AtmosphereConfig().to_params()
+    v = capi.f3((0.0, 1.0, 0.0))
+    fake = 16
+
+    def call(p, w, h, view=0, lut=fake):
+        return lib.atmo_composite_noalloc(
+            p, lut, fake, fake, fake, w, h, fake, fake, fake, fake, fake, 6.36e6, 10.0, 10.0, v,
+            v, v, 0.0046, v, v, v, 0.5, 4.0, 12.0, view, None)
+
+    assert call(ctypes.byref(params), 64, 64, lut=None) == capi.ATMO_EINVAL
+    assert call(None, 64, 64) == capi.ATMO_EINVAL
+    assert call(ctypes.byref(params), 64, 64, view=11) == capi.ATMO_EINVAL
+    assert call(ctypes.byref(params), 0, 64) == capi.ATMO_EINVAL
+    assert call(ctypes.byref(params), 4096, 2049) == capi.ATMO_ESHAPE
+    assert b"exceed" in lib.atmo_last_error(None)
+    bad = pkg.AtmosphereConfig().to_params()
+    bad.transmittance_steps = 1
+    assert call(ctypes.byref(bad), 64, 64) == capi.ATMO_EINVAL
+    assert b"transmittance_steps" in lib.atmo_last_error(None)
