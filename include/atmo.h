@@ -225,6 +225,123 @@ int32_t atmo_composite(
     float tangent_half_fov, float star_contrast_start,
     float star_contrast_end, int32_t diagnostic_view, void* stream);
 
+/* ------------------------------------------------------------------------
+ * The reference's host methods in C.  `AtmosphereRenderer.update`,
+ * `prepare_frame`, `composite` and `set_luts_frozen` (renderer.py:2057-2238)
+ * take float64 host values, decide from invalidation keys which LUTs to
+ * rebuild, and round every kernel argument to float32.  The entry points
+ * below do the same for a C host, with the same keys, freeze rule,
+ * snapshots and rebuild counters, so a host without Python reproduces the
+ * reference's rebuild semantics exactly.  The policy itself is exported as
+ * pure host functions over an AtmoLutCache (no device work), which a handle
+ * keeps internally for atmo_update / atmo_prepare_frame / atmo_composite_frame.
+ * The planet radius is a per-call argument, as in the reference, not a field
+ * of AtmoParams (AtmosphereConfig has none: the radius comes from
+ * PlanetModel, src/planet_renderer/planet.py).
+ * --------------------------------------------------------------------- */
+
+/* LUT bits: update's T, MS and SV, prepare_frame's AP. */
+enum {
+    ATMO_LUT_TRANSMITTANCE = 1,
+    ATMO_LUT_MULTI_SCATTERING = 2,
+    ATMO_LUT_SKY_VIEW = 4,
+    ATMO_LUT_AERIAL = 8
+};
+
+/* The LUT cache state of `AtmosphereRenderer` (renderer.py:97-116).  Keys
+ * compare with float64 ==, so a NaN key never matches. */
+typedef struct AtmoLutCache {
+    int32_t has_planet_key;            /* _planet_radius_key is not None */
+    int32_t has_multi_scattering_key;  /* _multi_scattering_key is not None */
+    int32_t has_sky_key;               /* _sky_key is not None */
+    int32_t luts_frozen;
+    double planet_key;                 /* planet radius (float64) */
+    double multi_scattering_key[5];    /* R, E[0..2] (float64), float32(radians(alpha)) */
+    double sky_key[6];                 /* float32 altitude, float32 sun cosine, E[0..2],
+                                          float32(radians(alpha)) */
+    int32_t has_sky_snapshot;          /* sky_snapshot_* are not None */
+    int32_t aerial_available;          /* _aerial_available */
+    double sky_snapshot_altitude_m;
+    double sky_snapshot_sun_cosine;
+    double aerial_snapshot_altitude_m;
+    int64_t transmittance_rebuilds;
+    int64_t multi_scattering_rebuilds;
+    int64_t sky_view_rebuilds;
+    int64_t aerial_rebuilds;
+} AtmoLutCache;
+
+/* What one `update` builds, with the float32 kernel arguments the
+ * reference derives (renderer.py:2082-2137). */
+typedef struct AtmoUpdatePlan {
+    int32_t rebuild;             /* ATMO_LUT_* bits, built in the order T, MS, SV */
+    float bottom_radius;         /* float32(R) */
+    float solar_irradiance[3];   /* float32(E) */
+    float sun_angular_radius;    /* float32(radians(alpha)) */
+    float camera_altitude;       /* float32(max(r - R, 0)): the sky-view altitude key */
+    float sun_zenith_cosine;     /* float32(clip(y of the normalised sun, -1, 1)) */
+} AtmoUpdatePlan;
+
+/* The state of a new AtmosphereRenderer: no keys, not frozen, no
+ * snapshots, counters at 0. */
+int32_t atmo_lut_cache_init(AtmoLutCache* cache);
+/* `set_luts_frozen` (renderer.py:2057-2070): unfreezing drops the sky key. */
+int32_t atmo_lut_cache_set_frozen(AtmoLutCache* cache, int32_t frozen);
+/* The decisions of `update` (renderer.py:2072-2137): updates the keys,
+ * snapshots and counters and fills `plan`.  force_mask (ATMO_LUT_* bits;
+ * no counterpart in the reference) treats those keys as changed: forcing T
+ * therefore rebuilds MS and SV too, as a new planet radius does, and a
+ * forced SV is built even while frozen. */
+int32_t atmo_lut_cache_update(AtmoLutCache* cache, double planet_radius_m,
+                              double camera_radius_m, const double sun_local[3],
+                              const double solar_irradiance[3],
+                              double sun_angular_radius_degrees, int32_t force_mask,
+                              AtmoUpdatePlan* plan);
+/* The AP decision of `prepare_frame` (renderer.py:2173-2187): *rebuild_aerial
+ * = 1 unless frozen with an AP already built (or when force_mask has
+ * ATMO_LUT_AERIAL); on a rebuild, the snapshot altitude r - R and the
+ * counter are updated. */
+int32_t atmo_lut_cache_prepare(AtmoLutCache* cache, double planet_radius_m,
+                               double camera_radius_m, int32_t force_mask,
+                               int32_t* rebuild_aerial);
+/* The sky-view altitude `composite` samples at (renderer.py:2210-2217): the
+ * snapshot altitude if one exists, else max(r - R, 0). */
+double atmo_lut_cache_sky_altitude(const AtmoLutCache* cache, double planet_radius_m,
+                                   double camera_radius_m);
+
+/* The handle's copy of the policy.  atmo_autotune and atmo_set_tuning drop
+ * its keys (the LUTs were rebuilt with other arguments or shapes). */
+int32_t atmo_get_lut_cache(AtmoHandle* handle, AtmoLutCache* out);
+int32_t atmo_set_luts_frozen(AtmoHandle* handle, int32_t frozen);
+
+/* `update` (renderer.py:2072-2137) on the handle's LUTs: the plan of
+ * atmo_lut_cache_update, then atmo_build_transmittance /
+ * _multi_scattering / _sky_view for its bits.  rebuilt (may be NULL)
+ * receives the bits. */
+int32_t atmo_update(AtmoHandle* handle, double planet_radius_m, double camera_radius_m,
+                    const double sun_local[3], const double solar_irradiance[3],
+                    double sun_angular_radius_degrees, int32_t force_mask, void* stream,
+                    int32_t* rebuilt);
+/* `prepare_frame` (renderer.py:2139-2187): surface lighting, then the AP
+ * LUT unless frozen.  view_basis is (right, view_up, forward), 9 values. */
+int32_t atmo_prepare_frame(AtmoHandle* handle, const float* position_view,
+                           const int32_t* surface_id, double planet_radius_m,
+                           double camera_radius_m, const double sun_local[3],
+                           const double solar_irradiance[3],
+                           double sun_angular_radius_degrees, const double view_basis[9],
+                           double tangent_half_fov, int32_t force_mask, void* stream,
+                           int32_t* rebuilt);
+/* `composite` (renderer.py:2189-2238): atmo_composite with the float64
+ * host arguments rounded as the reference rounds them and the sky-view LUT
+ * sampled at the snapshot altitude. */
+int32_t atmo_composite_frame(AtmoHandle* handle, const float* scene_hdr, const float* space_hdr,
+                             const int32_t* surface_id, const float* position_view, float* hdr,
+                             double planet_radius_m, double camera_radius_m,
+                             const double sun_local[3], const double solar_irradiance[3],
+                             const double sun_disk_radiance[3],
+                             double sun_angular_radius_degrees, const double view_basis[9],
+                             double tangent_half_fov, double star_contrast_start,
+                             double star_contrast_end, int32_t diagnostic_view, void* stream);
+
 /* No counterpart in the reference.  atmo_composite for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,
  * hosts that forbid hipMalloc): stateless -- no handle; the caller passes

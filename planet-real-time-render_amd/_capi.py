@@ -50,6 +50,16 @@ EXPORTED_SYMBOLS = (
     "atmo_prepare_surface_lighting",
     "atmo_composite",
     "atmo_composite_noalloc",
+    "atmo_lut_cache_init",
+    "atmo_lut_cache_set_frozen",
+    "atmo_lut_cache_update",
+    "atmo_lut_cache_prepare",
+    "atmo_lut_cache_sky_altitude",
+    "atmo_get_lut_cache",
+    "atmo_set_luts_frozen",
+    "atmo_update",
+    "atmo_prepare_frame",
+    "atmo_composite_frame",
     "atmo_batch_create",
     "atmo_batch_destroy",
     "atmo_batch_last_error",
@@ -103,6 +113,53 @@ class AtmoSurrogateTransform(ctypes.Structure):
         ("cosine_stretch", ctypes.c_float),
         ("exact_transmittance", ctypes.c_int32),
     ]
+
+
+LUT_TRANSMITTANCE = 1
+LUT_MULTI_SCATTERING = 2
+LUT_SKY_VIEW = 4
+LUT_AERIAL = 8
+
+
+class AtmoLutCache(ctypes.Structure):
+    """Mirror of struct AtmoLutCache (include/atmo.h)."""
+
+    _fields_ = [
+        ("has_planet_key", ctypes.c_int32),
+        ("has_multi_scattering_key", ctypes.c_int32),
+        ("has_sky_key", ctypes.c_int32),
+        ("luts_frozen", ctypes.c_int32),
+        ("planet_key", ctypes.c_double),
+        ("multi_scattering_key", ctypes.c_double * 5),
+        ("sky_key", ctypes.c_double * 6),
+        ("has_sky_snapshot", ctypes.c_int32),
+        ("aerial_available", ctypes.c_int32),
+        ("sky_snapshot_altitude_m", ctypes.c_double),
+        ("sky_snapshot_sun_cosine", ctypes.c_double),
+        ("aerial_snapshot_altitude_m", ctypes.c_double),
+        ("transmittance_rebuilds", ctypes.c_int64),
+        ("multi_scattering_rebuilds", ctypes.c_int64),
+        ("sky_view_rebuilds", ctypes.c_int64),
+        ("aerial_rebuilds", ctypes.c_int64),
+    ]
+
+
+class AtmoUpdatePlan(ctypes.Structure):
+    """Mirror of struct AtmoUpdatePlan (include/atmo.h)."""
+
+    _fields_ = [
+        ("rebuild", ctypes.c_int32),
+        ("bottom_radius", ctypes.c_float),
+        ("solar_irradiance", ctypes.c_float * 3),
+        ("sun_angular_radius", ctypes.c_float),
+        ("camera_altitude", ctypes.c_float),
+        ("sun_zenith_cosine", ctypes.c_float),
+    ]
+
+
+def d3(values) -> "ctypes.Array[ctypes.c_double]":
+    """Three float64 values for the host-method entry points."""
+    return (ctypes.c_double * 3)(*[float(v) for v in values])
 
 
 class AtmoParams(ctypes.Structure):
@@ -246,6 +303,22 @@ def _signatures(lib: ctypes.CDLL) -> None:
         _F, _F, _F, _I, _P,
     ]
     _PI = ctypes.POINTER(ctypes.c_int32)
+    _PD3 = ctypes.POINTER(ctypes.c_double)
+    _D = ctypes.c_double
+    _PC = ctypes.POINTER(AtmoLutCache)
+    lib.atmo_lut_cache_init.argtypes = [_PC]
+    lib.atmo_lut_cache_set_frozen.argtypes = [_PC, _I]
+    lib.atmo_lut_cache_update.argtypes = [_PC, _D, _D, _PD3, _PD3, _D, _I,
+                                          ctypes.POINTER(AtmoUpdatePlan)]
+    lib.atmo_lut_cache_prepare.argtypes = [_PC, _D, _D, _I, _PI]
+    lib.atmo_lut_cache_sky_altitude.argtypes = [_PC, _D, _D]
+    lib.atmo_lut_cache_sky_altitude.restype = _D
+    lib.atmo_get_lut_cache.argtypes = [_P, _PC]
+    lib.atmo_set_luts_frozen.argtypes = [_P, _I]
+    lib.atmo_update.argtypes = [_P, _D, _D, _PD3, _PD3, _D, _I, _P, _PI]
+    lib.atmo_prepare_frame.argtypes = [_P, _P, _P, _D, _D, _PD3, _PD3, _D, _PD3, _D, _I, _P, _PI]
+    lib.atmo_composite_frame.argtypes = [_P, _P, _P, _P, _P, _P, _D, _D, _PD3, _PD3, _PD3, _D,
+                                         _PD3, _D, _D, _D, _I, _P]
     lib.atmo_batch_create.argtypes = [ctypes.POINTER(AtmoBatchState), _I, _I, _I,
                                       ctypes.POINTER(_P), ctypes.POINTER(_P)]
     lib.atmo_batch_destroy.argtypes = [_P]
@@ -284,7 +357,7 @@ def _signatures(lib: ctypes.CDLL) -> None:
                                        ctypes.POINTER(AtmoShadeParams), _I, _I, _P]
     for name in EXPORTED_SYMBOLS:
         if name not in ("atmo_last_error", "atmo_mlp_last_error", "atmo_batch_last_error",
-                        "atmo_post_last_error"):
+                        "atmo_post_last_error", "atmo_lut_cache_sky_altitude"):
             getattr(lib, name).restype = ctypes.c_int32
 
 

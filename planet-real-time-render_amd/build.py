@@ -30,7 +30,7 @@ CSRC = PACKAGE_DIR / "csrc"
 TARGETS = {
     PACKAGE_DIR / "libatmo.so": [CSRC / "atmo_kernels.hip", CSRC / "atmo_robust.hip",
                                  CSRC / "surrogate.hip", CSRC / "postprocess.hip",
-                                 CSRC / "shade.hip"],
+                                 CSRC / "shade.hip", CSRC / "atmo_frame.hip"],
     PACKAGE_DIR / "libatmo_probe.so": [CSRC / "probe.hip"],
 }
 ARCH = "gfx950"

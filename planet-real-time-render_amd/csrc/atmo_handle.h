@@ -34,6 +34,7 @@ struct AtmoHandle {
     int precision;     // ATMO_PRECISION_REFERENCE or ATMO_PRECISION_ROBUST
     int group_t;       // lanes per transmittance texel (4, 8, 16), autotuned
     int group_sv;      // lanes per sky-view texel (4, 8, 16), autotuned
+    AtmoLutCache cache;  // the host policy of atmo_update / atmo_prepare_frame
     char error[512];
 };
 

@@ -260,6 +260,7 @@ int32_t atmo_create(const AtmoParams* params, int32_t width, int32_t height, Atm
     h->queue = nullptr;
     h->queue_parity = 0;
     h->queue_reset = false;
+    atmo_lut_cache_init(&h->cache);
     if (hipGetDevice(&h->device) != hipSuccess) {
         set_error(nullptr, "atmo_create: no HIP device");
         delete h;
@@ -716,6 +717,8 @@ int32_t atmo_autotune(AtmoHandle* h, float bottom, const float irradiance[3], fl
         if (ms < best) best = ms, gs = g;
     }
     h->group_sv = gs;
+    // the LUTs now hold this call's arguments: no cached key still applies
+    h->cache.has_planet_key = h->cache.has_multi_scattering_key = h->cache.has_sky_key = 0;
     // leave the sky-view LUT as the chosen variant builds it
     ATMO_STAGE(h, launch_sky_view, h->k, bottom, cam_alt, sun_zc, e, ar, h->tlut, h->mslut,
                h->svlut, s, gs);
@@ -742,6 +745,8 @@ int32_t atmo_set_tuning(AtmoHandle* h, const int32_t tuning[2]) {
     }
     h->group_t = tuning[0];
     h->group_sv = tuning[1];
+    // the next update rebuilds every LUT with the new shapes
+    h->cache.has_planet_key = h->cache.has_multi_scattering_key = h->cache.has_sky_key = 0;
     return ATMO_OK;
 }
 
