@@ -48,6 +48,19 @@ COST = {
     "sky_pixel": (328.0, 32.0),
     "refined_interval": (2946.0, 336.0),
 }
+# Algorithmic work of the LUT builds per state and of the surface lighting
+# per surface pixel at the full-frame configuration (T 256x64 / 40 steps, MS
+# 32x32 x 64 directions x 20 steps, SV 192x108 / 30 steps, AP 32x32x32 with 2
+# steps per slice), from the same hand count: (FP32 ops, transcendental ops).
+LUT_COST = {
+    "transmittance": (57.8e6, 2.1e6),
+    "multi_scattering": (430.6e6, 47.0e6),
+    "sky_view": (239.1e6, 25.1e6),
+    "aerial_perspective": (24.7e6, 2.6e6),
+}
+SURFACE_LIGHTING_COST = (242.0, 29.0)
+BATCH_STAGE_BITS = {"transmittance": 1, "multi_scattering": 2, "sky_view": 4,
+                    "aerial_perspective": 8}
 # Composite algorithmic bytes per pixel: surface_id (4 B) and two 12-byte
 # inputs read (scene + space for a sky pixel, scene + position for a surface
 # pixel), hdr (12 B) written.
@@ -225,8 +238,24 @@ def rel_err_stats(got, want, mask=None):
             "max": float(rel.max())}
 
 
+def stage_roofline(seconds: float, flop: float, tops: float, p_t: float | None) -> dict:
+    """Achieved FP32 and transcendental rates of one launch against the
+    FP32 vector peak (packed and non-packed) and the measured v_exp_f32
+    rate."""
+    tf = flop / seconds / 1e12
+    out = {"us": round(seconds * 1e6, 2), "algorithmic_flop": flop,
+           "algorithmic_transcendental": tops, "achieved_tflops": round(tf, 3),
+           "frac": round(tf / FP32_PEAK_TFLOPS, 4),
+           "frac_of_non_packed_peak": round(tf / FP32_NONPACKED_PEAK_TFLOPS, 4),
+           "transcendental_rate": round(tops / seconds / 1e12, 4)}
+    if p_t:
+        out["transcendental_frac"] = round(tops / seconds / p_t, 4)
+    return out
+
+
 def bench_batch(pkg, scene, device, steps: int):
-    """256 independent states' four LUTs per graph launch."""
+    """256 independent states' four LUTs per graph launch, then each stage
+    kernel timed on its own (direct launches of the same kernels)."""
     import importlib
     import torch
 
@@ -245,8 +274,19 @@ def bench_batch(pkg, scene, device, steps: int):
     e1.record(stream)
     torch.cuda.synchronize(device)
     ms = e0.elapsed_time(e1) / steps
+    stage_ms = {}
+    for name, bit in BATCH_STAGE_BITS.items():
+        b.run_stages(bit, stream)
+        torch.cuda.synchronize(device)
+        e0.record(stream)
+        for _ in range(steps):
+            b.run_stages(bit, stream)
+        e1.record(stream)
+        torch.cuda.synchronize(device)
+        stage_ms[name] = e0.elapsed_time(e1) / steps
     b.close()
     return {"states": len(states), "ms_per_batch": round(ms, 4),
+            "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
             "states_per_s": round(len(states) / ms * 1e3, 1),
             "lanes_per_texel_t_sv": list(tuning),
             "luts": "T 256x64, MS 32x32, SV 192x108, AP 32x32x32 per state; half Earth, "
@@ -669,6 +709,30 @@ def main() -> int:
         "hbm_gbs": round(bytes_per_launch / comp_s / 1e9, 2),
         "hbm_frac": round(bytes_per_launch / comp_s / 1e9 / HBM_PEAK_GBS, 5),
     }
+    # Roofline of every LUT build and of the surface lighting, single state
+    # (the plain frame's per-stage event medians) and, per stage kernel, the
+    # 256-state batch.
+    lut_roofline = {"single_state": {}, "batch_256": {}}
+    for name, (f_op, t_op) in LUT_COST.items():
+        lut_roofline["single_state"][name] = stage_roofline(stage_ms[name] * 1e-3, f_op, t_op,
+                                                            p_t)
+    lut_roofline["single_state"]["surface_lighting"] = stage_roofline(
+        stage_ms["surface_lighting"] * 1e-3, SURFACE_LIGHTING_COST[0] * n_surface,
+        SURFACE_LIGHTING_COST[1] * n_surface, p_t)
+    if batch:
+        count = batch["states"]
+        for name, (f_op, t_op) in LUT_COST.items():
+            lut_roofline["batch_256"][name] = stage_roofline(
+                batch["stage_ms"][name] * 1e-3, f_op * count, t_op * count, p_t)
+        total_f = sum(c[0] for c in LUT_COST.values()) * count
+        total_t = sum(c[1] for c in LUT_COST.values()) * count
+        lut_roofline["batch_256"]["graph"] = stage_roofline(batch["ms_per_batch"] * 1e-3,
+                                                            total_f, total_t, p_t)
+    for scope, rows in lut_roofline.items():
+        for name, r in rows.items():
+            log(f"roofline {scope:12s} {name:20s} {r['us']:9.2f} us {r['achieved_tflops']:7.2f} "
+                f"TFLOP/s ({100 * r['frac']:.1f}% of 157.3), transcendental "
+                f"{r['transcendental_rate']:.2f} T/s")
     config = {
         "workload": (f"{inputs.width}x{inputs.height} atmosphere frame: T 256x64/40, MS 32x32 "
                      "64 dirs x 20 steps, SV 192x108/30, AP 32x32x32, surface lighting, "
@@ -713,6 +777,7 @@ def main() -> int:
         "data": "synthetic",
         "config": config,
         "roofline": roofline,
+        "lut_roofline": lut_roofline,
         "cpu_baseline": cpu,
         "accuracy": accuracy,
         "parity": parity,

@@ -408,6 +408,12 @@ const char* atmo_batch_last_error(const AtmoBatch* batch);
 /* Builds all four LUTs of every state: one graph launch on `stream` (the
  * graph is captured on the first call). */
 int32_t atmo_batch_run(AtmoBatch* batch, void* stream);
+/* Launches a subset of the batch's four stage kernels directly on `stream`
+ * (no graph): stages is an OR of ATMO_LUT_TRANSMITTANCE (1),
+ * ATMO_LUT_MULTI_SCATTERING (2), ATMO_LUT_SKY_VIEW (4) and ATMO_LUT_AERIAL
+ * (8).  A stage reads the LUTs its predecessors left in the batch arrays,
+ * so after one atmo_batch_run any single stage can be timed on its own. */
+int32_t atmo_batch_run_stages(AtmoBatch* batch, int32_t stages, void* stream);
 /* Times the nine (G_T, G_SV) launch shapes of the batch's four stages on
  * `stream` (synchronising it), keeps the fastest for the captured graph
  * (capturing again on the next atmo_batch_run if it changed) and writes it

@@ -64,6 +64,7 @@ EXPORTED_SYMBOLS = (
     "atmo_batch_destroy",
     "atmo_batch_last_error",
     "atmo_batch_run",
+    "atmo_batch_run_stages",
     "atmo_batch_autotune",
     "atmo_batch_get_buffer",
     "atmo_mlp_create",
@@ -325,6 +326,7 @@ def _signatures(lib: ctypes.CDLL) -> None:
     lib.atmo_batch_last_error.argtypes = [_P]
     lib.atmo_batch_last_error.restype = ctypes.c_char_p
     lib.atmo_batch_run.argtypes = [_P, _P]
+    lib.atmo_batch_run_stages.argtypes = [_P, _I, _P]
     lib.atmo_batch_autotune.argtypes = [_P, _P, ctypes.POINTER(ctypes.c_int32)]
     lib.atmo_batch_get_buffer.argtypes = [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_int64)]
     lib.atmo_mlp_create.argtypes = [_I, ctypes.POINTER(_P)]

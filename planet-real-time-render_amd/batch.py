@@ -98,6 +98,18 @@ class LutBatch:
             raise _capi.AtmoError(self._lib.atmo_batch_last_error(self._handle)
                                   .decode(errors="replace"))
 
+    def run_stages(self, stages: int, stream=None) -> None:
+        """Launch only the stage kernels in `stages` (bits 1, 2, 4, 8: T, MS,
+        SV, AP) directly, without the graph; each reads what the previous
+        stages left in the batch arrays (atmo_batch_run_stages)."""
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        status = self._lib.atmo_batch_run_stages(self._handle, int(stages), s.cuda_stream)
+        if status != _capi.ATMO_OK:
+            raise _capi.AtmoError(self._lib.atmo_batch_last_error(self._handle)
+                                  .decode(errors="replace"))
+
     def autotune(self, stream=None) -> tuple[int, int]:
         """Time the nine (lanes per T texel, lanes per sky-view texel)
         launch shapes of the batch's stages and keep the fastest for the

@@ -943,6 +943,21 @@ int32_t atmo_batch_run(AtmoBatch* b, void* stream) {
     return ATMO_OK;
 }
 
+int32_t atmo_batch_run_stages(AtmoBatch* b, int32_t stages, void* stream) {
+    if (!batch_ok(b)) return ATMO_EHANDLE;
+    if (stages <= 0 || stages > 15) {
+        batch_error(b, "atmo_batch_run_stages: stages must be a non-empty subset of 1|2|4|8");
+        return ATMO_EINVAL;
+    }
+    atmo::ref::launch_batch(b->k0, b->states, (unsigned)b->count, b->width, b->height, b->luts,
+                            b->strides, (hipStream_t)stream, b->group_t, b->group_sv, stages);
+    if (hipGetLastError() != hipSuccess) {
+        batch_error(b, "atmo_batch_run_stages: launch failed");
+        return ATMO_EHIP;
+    }
+    return ATMO_OK;
+}
+
 int32_t atmo_batch_autotune(AtmoBatch* b, void* stream, int32_t chosen[2]) {
     if (!batch_ok(b)) return ATMO_EHANDLE;
     hipStream_t s = (hipStream_t)stream;

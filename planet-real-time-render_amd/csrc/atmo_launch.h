@@ -41,8 +41,10 @@ namespace atmo {
 ATMO_REF_NAMESPACE {
 ATMO_DECLARE_LAUNCHERS
 // Reference precision only: the batched LUT builds.
+// `stages`: bits 1, 2, 4, 8 select the T, MS, SV and AP launches.
 void launch_batch(const Consts& k, const BatchState* states, unsigned count, int width, int height,
-                  float* const luts[4], const long strides[4], hipStream_t s, int g_t, int g_sv);
+                  float* const luts[4], const long strides[4], hipStream_t s, int g_t, int g_sv,
+                  int stages = 15);
 }  // namespace ref
 ATMO_ROBUST_NAMESPACE {
 ATMO_DECLARE_LAUNCHERS

@@ -762,40 +762,47 @@ void launch_composite(const Consts& k, const CompositeArgs& a, hipStream_t s) {
 #if !ATMO_ROBUST
 template <int GT, int GS>
 void launch_batch_g(const Consts& k, const BatchState* states, unsigned count, int width,
-                    int height, float* const luts[4], const long strides[4], hipStream_t s) {
-    hipLaunchKernelGGL(k_transmittance_batched<GT>, dim3(grid_for((long)k.tw * k.th * GT), count),
-                       dim3(256), 0, s, states, luts[0], strides[0]);
-    hipLaunchKernelGGL(k_multi_scattering_batched, dim3((unsigned)(k.mw * k.mh), count),
-                       dim3(256), 0, s, states, (const float*)luts[0], strides[0], luts[1],
-                       strides[1]);
-    hipLaunchKernelGGL(k_sky_view_batched<GS>, dim3(grid_for((long)k.sw * k.sh * GS), count),
-                       dim3(256), 0, s, states, (const float*)luts[0], strides[0],
-                       (const float*)luts[1], strides[1], luts[2], strides[2]);
-    hipLaunchKernelGGL(k_aerial_batched, dim3(grid_for((long)k.aw * k.ah * 64), count), dim3(256), 0,
-                       s, states, width, height, (const float*)luts[0], strides[0],
-                       (const float*)luts[1], strides[1], luts[3], strides[3]);
+                    int height, float* const luts[4], const long strides[4], hipStream_t s,
+                    int stages) {
+    if (stages & 1)
+        hipLaunchKernelGGL(k_transmittance_batched<GT>,
+                           dim3(grid_for((long)k.tw * k.th * GT), count), dim3(256), 0, s, states,
+                           luts[0], strides[0]);
+    if (stages & 2)
+        hipLaunchKernelGGL(k_multi_scattering_batched, dim3((unsigned)(k.mw * k.mh), count),
+                           dim3(256), 0, s, states, (const float*)luts[0], strides[0], luts[1],
+                           strides[1]);
+    if (stages & 4)
+        hipLaunchKernelGGL(k_sky_view_batched<GS>, dim3(grid_for((long)k.sw * k.sh * GS), count),
+                           dim3(256), 0, s, states, (const float*)luts[0], strides[0],
+                           (const float*)luts[1], strides[1], luts[2], strides[2]);
+    if (stages & 8)
+        hipLaunchKernelGGL(k_aerial_batched, dim3(grid_for((long)k.aw * k.ah * 64), count),
+                           dim3(256), 0, s, states, width, height, (const float*)luts[0],
+                           strides[0], (const float*)luts[1], strides[1], luts[3], strides[3]);
 }
 
 template <int GT>
 void launch_batch_gt(const Consts& k, const BatchState* states, unsigned count, int width,
                      int height, float* const luts[4], const long strides[4], hipStream_t s,
-                     int g_sv) {
+                     int g_sv, int stages) {
     if (g_sv == 4)
-        launch_batch_g<GT, 4>(k, states, count, width, height, luts, strides, s);
+        launch_batch_g<GT, 4>(k, states, count, width, height, luts, strides, s, stages);
     else if (g_sv == 16)
-        launch_batch_g<GT, 16>(k, states, count, width, height, luts, strides, s);
+        launch_batch_g<GT, 16>(k, states, count, width, height, luts, strides, s, stages);
     else
-        launch_batch_g<GT, 8>(k, states, count, width, height, luts, strides, s);
+        launch_batch_g<GT, 8>(k, states, count, width, height, luts, strides, s, stages);
 }
 
 void launch_batch(const Consts& k, const BatchState* states, unsigned count, int width, int height,
-                  float* const luts[4], const long strides[4], hipStream_t s, int g_t, int g_sv) {
+                  float* const luts[4], const long strides[4], hipStream_t s, int g_t, int g_sv,
+                  int stages) {
     if (g_t == 4)
-        launch_batch_gt<4>(k, states, count, width, height, luts, strides, s, g_sv);
+        launch_batch_gt<4>(k, states, count, width, height, luts, strides, s, g_sv, stages);
     else if (g_t == 16)
-        launch_batch_gt<16>(k, states, count, width, height, luts, strides, s, g_sv);
+        launch_batch_gt<16>(k, states, count, width, height, luts, strides, s, g_sv, stages);
     else
-        launch_batch_gt<8>(k, states, count, width, height, luts, strides, s, g_sv);
+        launch_batch_gt<8>(k, states, count, width, height, luts, strides, s, g_sv, stages);
 }
 #endif
 

@@ -116,6 +116,24 @@ def test_batch_graph_replays_are_stable(built, device):
             assert torch.equal(t, f)
 
 
+def test_batch_stage_launches_equal_the_graph(built, device):
+    """atmo_batch_run_stages: the four stage kernels launched one by one, in
+    order, give the graph's LUTs bit for bit."""
+    import torch
+
+    states, b = built
+    b.run()
+    torch.cuda.synchronize(device)
+    first = [t.clone() for t in b.luts]
+    for t in b.luts:
+        t.fill_(-1.0)
+    for bit in (1, 2, 4, 8):
+        b.run_stages(bit)
+    torch.cuda.synchronize(device)
+    for t, f in zip(b.luts, first):
+        assert torch.equal(t, f)
+
+
 def test_batch_luts_satisfy_reference_invariants(built):
     """tests/test_m2_terrain.py:252-270: T in [0, 1]; MS, SV, AP finite and
     non-negative with a positive maximum, per state."""
