@@ -100,24 +100,50 @@ __device__ __forceinline__ float horizon_cosine(float R, float h) {
 // cosine c_h and stretched as asinh((c - c_h) / s) / asinh(2 / s), so the
 // horizon kink sits at the same feature value at every altitude), relative
 // azimuth, log segment length.
-__device__ __forceinline__ void encode(const AtmoSurrogateTransform& t, float h, float mu,
-                                       float mu_s, float phi, float d, float x[5]) {
-    x[0] = log1pf(fmaxf(h, 0.0f) / t.altitude_scale_m) *
+// The per-sample constants of the encoding at altitude h.
+struct Encoding {
+    float x0;      // altitude feature
+    float ch;      // local horizon cosine
+    float inv;     // 1 / cosine_stretch
+    float norm;    // 1 / asinh(2 / cosine_stretch)
+    float ld0;     // log(distance_min)
+    float dscale;  // 2 / (log(distance_max) - log(distance_min))
+};
+
+__device__ __forceinline__ Encoding encoding_at(const AtmoSurrogateTransform& t, float h) {
+    Encoding e;
+    e.x0 = log1pf(fmaxf(h, 0.0f) / t.altitude_scale_m) *
                (2.0f / log1pf(t.altitude_max_m / t.altitude_scale_m)) -
            1.0f;
-    if (t.cosine_stretch > 0.0f) {
-        const float ch = horizon_cosine(t.planet_radius_m, h);
-        const float inv = 1.0f / t.cosine_stretch;
-        const float norm = 1.0f / asinhf(2.0f * inv);
-        x[1] = asinhf((mu - ch) * inv) * norm;
-        x[2] = asinhf((mu_s - ch) * inv) * norm;
-    } else {
-        x[1] = mu;
-        x[2] = mu_s;
-    }
-    x[3] = phi * (2.0f / 3.14159265358979f) - 1.0f;
-    float ld0 = logf(t.distance_min_m), ld1 = logf(t.distance_max_m);
-    x[4] = (logf(d) - ld0) * (2.0f / (ld1 - ld0)) - 1.0f;
+    e.ch = horizon_cosine(t.planet_radius_m, h);
+    e.inv = t.cosine_stretch > 0.0f ? 1.0f / t.cosine_stretch : 0.0f;
+    e.norm = t.cosine_stretch > 0.0f ? 1.0f / asinhf(2.0f * e.inv) : 0.0f;
+    e.ld0 = logf(t.distance_min_m);
+    e.dscale = 2.0f / (logf(t.distance_max_m) - e.ld0);
+    return e;
+}
+
+__device__ __forceinline__ float feature_cosine(const AtmoSurrogateTransform& t, const Encoding& e,
+                                                float c) {
+    return t.cosine_stretch > 0.0f ? asinhf((c - e.ch) * e.inv) * e.norm : c;
+}
+
+__device__ __forceinline__ float feature_azimuth(float phi) {
+    return phi * (2.0f / 3.14159265358979f) - 1.0f;
+}
+
+__device__ __forceinline__ float feature_distance(const Encoding& e, float d) {
+    return (logf(d) - e.ld0) * e.dscale - 1.0f;
+}
+
+__device__ __forceinline__ void encode(const AtmoSurrogateTransform& t, float h, float mu,
+                                       float mu_s, float phi, float d, float x[5]) {
+    const Encoding e = encoding_at(t, h);
+    x[0] = e.x0;
+    x[1] = feature_cosine(t, e, mu);
+    x[2] = feature_cosine(t, e, mu_s);
+    x[3] = feature_azimuth(phi);
+    x[4] = feature_distance(e, d);
 }
 
 __device__ __forceinline__ bool in_domain(const AtmoSurrogateTransform& t, float h, float d) {
@@ -323,40 +349,77 @@ __device__ __forceinline__ bf16x4 pack_bf16(floatx4 v) {
 }
 
 __host__ __device__ constexpr int frag_index(int layer, int mt, int kt) {
-    // layers 1-2: 4 x 4, layer 3: 1 x 4 bf16 fragments (layer 0 runs in fp32)
-    return layer == 1 ? mt * 4 + kt : layer == 2 ? 16 + mt * 4 + kt : 32 + kt;
+    // layers 1-2: 4 row tiles x 2 K-steps of 32, layer 3: 1 x 2 bf16
+    // fragments (layer 0 runs in fp32)
+    return layer == 1 ? mt * 2 + kt : layer == 2 ? 8 + mt * 2 + kt : 16 + kt;
 }
-constexpr int kFragments = 36;
-constexpr int kFragments32 = 8;  // layer 0: 4 row tiles x 2 K-steps of 4
+constexpr int kFragments = 18;
+// fp32 fragments of layer 0: [0, 8) the 5-input layer (4 row tiles x 2
+// K-steps of 4); [8, 12) the 3 per-pixel inputs (view cosine, azimuth,
+// distance) of the composite, whose altitude and sun cosine are frame
+// constants folded into the bias.
+constexpr int kFragments32 = 12;
+constexpr int kFolded32 = 8;
+// 16-pixel chunks the surrogate composite's MLP runs at once (1, 2 or 4).
+#ifndef ATMO_SURROGATE_CHUNKS
+#define ATMO_SURROGATE_CHUNKS 1
+#endif
+// Persistent blocks of the surrogate composite (at most; one per 256-pixel
+// group on small frames).
+#ifndef ATMO_SURROGATE_GRID
+#define ATMO_SURROGATE_GRID 2048
+#endif
+
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+
+// The neuron a K-step position holds.  The output tile mt of a layer leaves
+// lane l with neurons 16 mt + 4 (l / 16) + i (i = 0..3) of pixel l % 16;
+// the B operand of K-step kt of v_mfma_f32_16x16x32_bf16 takes 8 values per
+// lane at K positions 8 (l / 16) + i'.  Lane l's 8 values are the outputs
+// of tiles 2 kt and 2 kt + 1, so K position 8 g + i' stands for neuron
+// 32 kt + 4 g + i' (i' < 4) or 32 kt + 16 + 4 g + i' - 4 (i' >= 4); the
+// weights are packed with the same permutation, which leaves the sum over K
+// unchanged and chains the layers with no data movement.
+__host__ __device__ constexpr int k_neuron(int kt, int g, int i) {
+    return 32 * kt + (i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4));
+}
 
 // Packs fp32 parameters into per-lane A-operand fragments.  Layers 1-3,
-// v_mfma_f32_16x16x16bf16_1k: for tile (mt, kt) lane l holds
-// W[16 mt + l % 16][16 kt + 4 (l / 16) + i], i = 0..3, in bf16 (zero outside
-// W).  Layer 0, v_mfma_f32_16x16x4f32: lane l holds W[16 mt + l % 16][4 kt +
-// l / 16] in fp32.  Also gathers the biases in output-row order.
+// v_mfma_f32_16x16x32_bf16: for tile (mt, kt) lane l holds
+// W[16 mt + l % 16][k_neuron(kt, l / 16, i)], i = 0..7, in bf16 (zero
+// outside W).  Layer 0, v_mfma_f32_16x16x4f32: general form, lane l holds
+// W[16 mt + l % 16][4 kt + l / 16] in fp32; folded form, lane l holds
+// W[16 mt + l % 16][c] with c = 1, 3, 4 for l / 16 = 0, 1, 2 (0 for 3).
+// Also gathers the biases in output-row order.
 __global__ void k_pack_fragments(const float* __restrict__ params, uint16_t* __restrict__ frag,
                                  float* __restrict__ frag32, float* __restrict__ bias) {
     const int lane = threadIdx.x;  // 64 threads
     for (int layer = 1; layer < 4; ++layer) {
         const Layer L = kLayers[layer];
-        const int mts = (L.out + 15) / 16, kts = (L.in + 15) / 16;
+        const int mts = (L.out + 15) / 16, kts = L.in / 32;
         for (int mt = 0; mt < mts; ++mt)
             for (int kt = 0; kt < kts; ++kt) {
-                uint16_t* f = frag + ((long)frag_index(layer, mt, kt) * 64 + lane) * 4;
+                uint16_t* f = frag + ((long)frag_index(layer, mt, kt) * 64 + lane) * 8;
                 const int row = 16 * mt + lane % 16;
-                for (int i = 0; i < 4; ++i) {
-                    const int col = 16 * kt + 4 * (lane / 16) + i;
+                for (int i = 0; i < 8; ++i) {
+                    const int col = k_neuron(kt, lane / 16, i);
                     float w = (row < L.out && col < L.in) ? params[L.w + row * L.in + col] : 0.0f;
                     f[i] = (uint16_t)to_bf16(w);
                 }
             }
     }
     const Layer L0 = kLayers[0];
-    for (int mt = 0; mt < 4; ++mt)
+    for (int mt = 0; mt < 4; ++mt) {
+        const int row = 16 * mt + lane % 16;
         for (int kt = 0; kt < 2; ++kt) {
-            const int row = 16 * mt + lane % 16, col = 4 * kt + lane / 16;
+            const int col = 4 * kt + lane / 16;
             frag32[(mt * 2 + kt) * 64 + lane] = col < L0.in ? params[L0.w + row * L0.in + col] : 0.0f;
         }
+        const int g = lane / 16;
+        const int col = g == 0 ? 1 : g == 1 ? 3 : 4;
+        frag32[(kFolded32 + mt) * 64 + lane] = g < 3 ? params[L0.w + row * L0.in + col] : 0.0f;
+    }
     for (int layer = 0; layer < 4; ++layer) {
         const Layer L = kLayers[layer];
         for (int r = lane; r < (layer < 3 ? 64 : 16); r += 64)
@@ -364,8 +427,10 @@ __global__ void k_pack_fragments(const float* __restrict__ params, uint16_t* __r
     }
 }
 
-__device__ __forceinline__ floatx4 mfma(bf16x4 a, bf16x4 b, floatx4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+__device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x4 lo, bf16x4 hi, floatx4 c) {
+    const bf16x8 b = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8v, a),
+                                                    __builtin_bit_cast(bf16x8v, b), c, 0, 0, 0);
 }
 
 // SiLU z / (1 + e^-z) at hardware rate (v_exp_f32, v_rcp_f32), then
@@ -380,20 +445,62 @@ __device__ __forceinline__ bf16x4 silu_bf16(floatx4 c) {
     return pack_bf16(r);
 }
 
+// Layers 2-4 on the layer-1 activations of 16 pixels: act[mt] holds lane
+// l's neurons 16 mt + 4 (l / 16) + i of pixel l % 16.  Returns layer 4's
+// output tile (lane l: outputs 4 (l / 16) + i of pixel l % 16).
+__device__ __forceinline__ floatx4 mlp_tail(bf16x4 act[4], const bf16x8* __restrict__ frag,
+                                            const float* __restrict__ bias, int lane) {
+    const int group = lane / 16;
+#pragma unroll
+    for (int layer = 1; layer <= 2; ++layer) {
+        bf16x4 next[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            floatx4 c;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) c[i] = bias[64 * layer + 16 * mt + 4 * group + i];
+            c = mfma(frag[frag_index(layer, mt, 0) * 64 + lane], act[0], act[1], c);
+            c = mfma(frag[frag_index(layer, mt, 1) * 64 + lane], act[2], act[3], c);
+            next[mt] = silu_bf16(c);
+        }
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) act[mt] = next[mt];
+    }
+    floatx4 c;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c[i] = bias[192 + 4 * group + i];
+    c = mfma(frag[frag_index(3, 0, 0) * 64 + lane], act[0], act[1], c);
+    return mfma(frag[frag_index(3, 0, 1) * 64 + lane], act[2], act[3], c);
+}
+
+// The 6 raw outputs of pixel 16 j + col from layer 4's tile: outputs 0-3 sit
+// in lane col, 4-5 in lane col + 16; lane 16 j + col takes them.
+__device__ __forceinline__ void gather_outputs(floatx4 c, int j, int lane, float out[6]) {
+    const int group = lane / 16, col = lane % 16;
+    float o[6];
+    o[0] = __shfl(c[0], col, 64);
+    o[1] = __shfl(c[1], col, 64);
+    o[2] = __shfl(c[2], col, 64);
+    o[3] = __shfl(c[3], col, 64);
+    o[4] = __shfl(c[0], col + 16, 64);
+    o[5] = __shfl(c[1], col + 16, 64);
+    if (group == j)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) out[q] = o[q];
+}
+
 // Runs the MLP for the 64 pixels of a wave.  `x` holds the lane's own
 // pixel features; returns the 6 raw outputs of the lane's pixel.  The wave
 // walks its pixels 16 at a time.  Layer 1 (5 -> 64) runs on
 // v_mfma_f32_16x16x4f32 in full fp32 (bf16 inputs would quantise log(d) by
 // ~0.4 %, i.e. the segment length by ~3 %): lane l feeds feature
 // 4 kt + l / 16 of pixel l % 16 as the B operand.  Layers 2-4 run on
-// v_mfma_f32_16x16x16bf16_1k with the weights as the A operand and the
-// activations as the B operand, so the output tile of one layer (lane l:
-// neurons 4 (l / 16) + i of pixel l % 16) is exactly the B fragment of the
-// next layer's K-step: no data movement between layers.  `a32` holds the
-// lane's 8 fp32 layer-1 fragments, `bias` the fp32 biases of all layers.
-__device__ __forceinline__ void mlp_wave(const float x[5], const bf16x4* __restrict__ frag,
-                                         const float a32[kFragments32],
-                                         const float* __restrict__ bias, float out[6]) {
+// gfx950's v_mfma_f32_16x16x32_bf16 with the weights as the A operand and
+// the activations as the B operand (see k_neuron).  `a32` holds the lane's
+// 8 fp32 layer-1 fragments, `bias` the fp32 biases of all layers.
+__device__ __forceinline__ void mlp_wave(const float x[5], const bf16x8* __restrict__ frag,
+                                         const float a32[8], const float* __restrict__ bias,
+                                         float out[6]) {
     const int lane = threadIdx.x % 64;
     const int group = lane / 16, col = lane % 16;
 #pragma unroll 1
@@ -414,38 +521,83 @@ __device__ __forceinline__ void mlp_wave(const float x[5], const bf16x4* __restr
             c = __builtin_amdgcn_mfma_f32_16x16x4f32(a32[2 * mt + 1], b1, c, 0, 0, 0);
             act[mt] = silu_bf16(c);
         }
+        gather_outputs(mlp_tail(act, frag, bias, lane), j, lane, out);
+    }
+}
+
+// mlp_wave for pixels of one frame, whose altitude and sun-cosine features
+// are constants already folded into layer 1's bias (`bias0`): the lane
+// feeds its pixel's view-cosine, azimuth and distance features (x1, x3,
+// x4), and layer 1 is one K-step per row tile with the folded fragments
+// `a32f`.  Layer-major: each layer runs for C 16-pixel chunks at once, so a
+// wave has 4 C independent MFMA / SiLU chains in flight and each weight
+// fragment is read from LDS once per C chunks; the wave's 64 pixels take
+// 4 / C rounds.
+template <int C>
+__device__ __forceinline__ void mlp_wave_folded(float x1, float x3, float x4,
+                                                const bf16x8* __restrict__ frag,
+                                                const float a32f[4],
+                                                const float* __restrict__ bias0,
+                                                const float* __restrict__ bias, float out[6]) {
+    const int lane = threadIdx.x % 64;
+    const int group = lane / 16, col = lane % 16;
+#pragma unroll 1
+    for (int round = 0; round < 4 / C; ++round) {
+        asm volatile("" ::: "memory");
+        bf16x4 act[C][4];
 #pragma unroll
-        for (int layer = 1; layer <= 2; ++layer) {
-            bf16x4 next[4];
+        for (int q = 0; q < C; ++q) {
+            const int src = 16 * (round * C + q) + col;
+            const float f1 = __shfl(x1, src, 64), f3 = __shfl(x3, src, 64);
+            const float f4 = __shfl(x4, src, 64);
+            const float b = group == 0 ? f1 : group == 1 ? f3 : group == 2 ? f4 : 0.0f;
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) {
                 floatx4 c;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) c[i] = bias[64 * layer + 16 * mt + 4 * group + i];
+                for (int i = 0; i < 4; ++i) c[i] = bias0[16 * mt + 4 * group + i];
+                c = __builtin_amdgcn_mfma_f32_16x16x4f32(a32f[mt], b, c, 0, 0, 0);
+                act[q][mt] = silu_bf16(c);
+            }
+        }
 #pragma unroll
-                for (int kt = 0; kt < 4; ++kt)
-                    c = mfma(frag[frag_index(layer, mt, kt) * 64 + lane], act[kt], c);
-                next[mt] = silu_bf16(c);
+        for (int layer = 1; layer <= 2; ++layer) {
+            // The weights and biases are re-read from LDS here rather than
+            // held in registers across the pixel loop: an empty asm with a
+            // memory clobber keeps the compiler from hoisting the loads.
+            asm volatile("" ::: "memory");
+            bf16x4 next[C][4];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                const bf16x8 w0 = frag[frag_index(layer, mt, 0) * 64 + lane];
+                const bf16x8 w1 = frag[frag_index(layer, mt, 1) * 64 + lane];
+                floatx4 b;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) b[i] = bias[64 * layer + 16 * mt + 4 * group + i];
+#pragma unroll
+                for (int q = 0; q < C; ++q) {
+                    floatx4 c = mfma(w0, act[q][0], act[q][1], b);
+                    c = mfma(w1, act[q][2], act[q][3], c);
+                    next[q][mt] = silu_bf16(c);
+                }
             }
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt) act[mt] = next[mt];
+            for (int q = 0; q < C; ++q)
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) act[q][mt] = next[q][mt];
         }
-        floatx4 c;
+        asm volatile("" ::: "memory");
+        const bf16x8 w0 = frag[frag_index(3, 0, 0) * 64 + lane];
+        const bf16x8 w1 = frag[frag_index(3, 0, 1) * 64 + lane];
+        floatx4 b;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) c[i] = bias[192 + 4 * group + i];
+        for (int i = 0; i < 4; ++i) b[i] = bias[192 + 4 * group + i];
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) c = mfma(frag[frag_index(3, 0, kt) * 64 + lane], act[kt], c);
-        // outputs 0-3 of pixel 16 j + col sit in lane col, 4-5 in lane col + 16
-        float o[6];
-        o[0] = __shfl(c[0], col, 64);
-        o[1] = __shfl(c[1], col, 64);
-        o[2] = __shfl(c[2], col, 64);
-        o[3] = __shfl(c[3], col, 64);
-        o[4] = __shfl(c[0], col + 16, 64);
-        o[5] = __shfl(c[1], col + 16, 64);
-        if (group == j)
-#pragma unroll
-            for (int q = 0; q < 6; ++q) out[q] = o[q];
+        for (int q = 0; q < C; ++q) {
+            floatx4 c = mfma(w0, act[q][0], act[q][1], b);
+            c = mfma(w1, act[q][2], act[q][3], c);
+            gather_outputs(c, round * C + q, lane, out);
+        }
     }
 }
 
@@ -497,6 +649,7 @@ struct SurrogateArgs {
     const uint16_t* frag;
     const float* frag32;
     const float* bias;
+    const float* params;  // fp32 master parameters (layer 1's weights for the bias fold)
     AtmoSurrogateTransform t;
 };
 
@@ -504,26 +657,35 @@ struct SurrogateArgs {
 // the surface branch's in-scatter (and, unless exact_transmittance is set,
 // transmittance); sky pixels are left to the plain sky pass.  Persistent
 // blocks: the 18 KiB of bf16 weight fragments are staged in LDS once per
-// block, which then walks 256-pixel groups.  In a group, surface pixels are
-// compacted onto the leading threads, so the network runs in full waves
-// over surface pixels only (the MFMA chain needs the whole wave; a wave runs
-// it when any lane holds an in-domain pixel).  Out-of-domain surface pixels
-// are queued for k_composite_list, which keeps the plain integrator's
-// registers out of this kernel.
+// block, with layer 1's bias folded with the frame's constant altitude and
+// sun-cosine features, and the block then walks 256-pixel groups.  In a
+// group, surface pixels are compacted onto the leading threads, so the
+// network runs in full waves over surface pixels only (the MFMA chain needs
+// the whole wave; a wave runs it when any lane holds an in-domain pixel).
+// Out-of-domain surface pixels are queued for k_composite_list, which keeps
+// the plain integrator's registers out of this kernel.
 __global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, SurrogateArgs a,
                                                              int* __restrict__ queue, int parity) {
-    __shared__ bf16x4 frag_s[kFragments * 64];
+    __shared__ bf16x8 frag_s[kFragments * 64];
     __shared__ float bias_s[64 * 3 + 16];
+    __shared__ float bias0_s[64];
     __shared__ int list[256];
     __shared__ int wave_counts[4];
-    const bf16x4* fg = reinterpret_cast<const bf16x4*>(a.frag);
+    const bf16x8* fg = reinterpret_cast<const bf16x8*>(a.frag);
     for (int e = threadIdx.x; e < kFragments * 64; e += blockDim.x) frag_s[e] = fg[e];
     for (int e = threadIdx.x; e < 64 * 3 + 16; e += blockDim.x) bias_s[e] = a.bias[e];
+    const Encoding enc = encoding_at(a.t, a.cam_alt);
+    if (threadIdx.x < 64) {
+        // z = W x + b with x0 (altitude) and x2 (sun cosine) fixed per frame
+        const float x2 = feature_cosine(a.t, enc, clampf(a.sun.y, -1.0f, 1.0f));
+        const float* w = a.params + kLayers[0].w + threadIdx.x * 5;
+        bias0_s[threadIdx.x] = (a.bias[threadIdx.x] + w[0] * enc.x0) + w[2] * x2;
+    }
     // (the first compaction's barriers order this staging before any read)
     const int lane = threadIdx.x % 64;
-    float a32[kFragments32];
+    float a32f[4];
 #pragma unroll
-    for (int f = 0; f < kFragments32; ++f) a32[f] = a.frag32[f * 64 + lane];
+    for (int f = 0; f < 4; ++f) a32f[f] = a.frag32[(kFolded32 + f) * 64 + lane];
 
     const long npix = (long)a.width * a.height;
     const long groups = (npix + 255) / 256;
@@ -541,17 +703,17 @@ __global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, Surrogate
         // queue the out-of-domain surface pixels
         enqueue_wave(has && !learned, j, queue, parity, npix);
         if (__ballot(learned)) {  // wave-uniform branch
-            float x[5];
             // relative azimuth as in the sky-view sampler (renderer.py:1482-1497)
             float vl = sqrtf(ray.x * ray.x + ray.z * ray.z);
             float sl = sqrtf(a.sun.x * a.sun.x + a.sun.z * a.sun.z);
             float phi = 0.0f;
             if (vl > 1.0e-6f && sl > 1.0e-6f)
                 phi = acosf(clampf((ray.x * a.sun.x + ray.z * a.sun.z) / (vl * sl), -1.0f, 1.0f));
-            encode(a.t, a.cam_alt, clampf(ray.y, -1.0f, 1.0f), clampf(a.sun.y, -1.0f, 1.0f), phi,
-                   fmaxf(sd, a.t.distance_min_m), x);
             float raw[6];
-            mlp_wave(x, frag_s, a32, bias_s, raw);
+            mlp_wave_folded<ATMO_SURROGATE_CHUNKS>(feature_cosine(a.t, enc, clampf(ray.y, -1.0f, 1.0f)),
+                            feature_azimuth(phi),
+                            feature_distance(enc, fmaxf(sd, a.t.distance_min_m)), frag_s, a32f,
+                            bias0_s, bias_s, raw);
             if (learned) {
                 f3 L, T;
                 decode(a.t, raw, L, T);
@@ -564,8 +726,6 @@ __global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, Surrogate
     }
 }
 
-
-
 // Batch inference on encoded features (testing and evaluation): x [n, 5]
 // -> y [n, 6] raw network outputs, through the same MFMA path.
 __global__ void __launch_bounds__(256) k_mlp_infer(long n, const float* __restrict__ x,
@@ -573,15 +733,15 @@ __global__ void __launch_bounds__(256) k_mlp_infer(long n, const float* __restri
                                                    const float* __restrict__ frag32,
                                                    const float* __restrict__ bias,
                                                    float* __restrict__ y) {
-    __shared__ bf16x4 frag_s[kFragments * 64];
+    __shared__ bf16x8 frag_s[kFragments * 64];
     __shared__ float bias_s[64 * 3 + 16];
-    const bf16x4* fg = reinterpret_cast<const bf16x4*>(frag);
+    const bf16x8* fg = reinterpret_cast<const bf16x8*>(frag);
     for (int e = threadIdx.x; e < kFragments * 64; e += blockDim.x) frag_s[e] = fg[e];
     for (int e = threadIdx.x; e < 64 * 3 + 16; e += blockDim.x) bias_s[e] = bias[e];
     __syncthreads();
     const int lane = threadIdx.x % 64;
-    float a32[kFragments32];
-    for (int f = 0; f < kFragments32; ++f) a32[f] = frag32[f * 64 + lane];
+    float a32[8];
+    for (int f = 0; f < 8; ++f) a32[f] = frag32[f * 64 + lane];
     const long i_raw = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long i = i_raw < n ? i_raw : n - 1;
     float f[5];
@@ -668,7 +828,7 @@ int32_t atmo_mlp_create(int32_t max_batch, AtmoMlp** out) {
               alloc(&m->wbuf, B * 6) == hipSuccess &&
               alloc(&m->partial, chunks * (64 * 64 + 64)) == hipSuccess &&
               alloc(&m->loss, B * 6 / 256 + 2) == hipSuccess &&
-              alloc(&m->frag, (size_t)kFragments * 64 * 4) == hipSuccess &&
+              alloc(&m->frag, (size_t)kFragments * 64 * 8) == hipSuccess &&
               alloc(&m->frag32, (size_t)kFragments32 * 64) == hipSuccess &&
               alloc(&m->bias, (size_t)64 * 3 + 16) == hipSuccess;
     for (int l = 0; l < 4 && ok; ++l) {
@@ -877,6 +1037,7 @@ int32_t atmo_composite_surrogate(AtmoHandle* h, AtmoMlp* m, const float* scene_h
     a.frag = m->frag;
     a.frag32 = m->frag32;
     a.bias = m->bias;
+    a.params = m->params;
     a.t = m->transform;
     const long npix = (long)h->width * h->height;
     const long groups = (npix + 255) / 256;
@@ -913,7 +1074,8 @@ int32_t atmo_composite_surrogate(AtmoHandle* h, AtmoMlp* m, const float* scene_h
     if (atmo_begin_queue(h, ca, s) != ATMO_OK) return ATMO_EHIP;
     h->queue_reset = true;  // until every launch of this call has gone out
     if (atmo_launch_composite_sky(h, ca, s) != ATMO_OK) return ATMO_EHIP;
-    hipLaunchKernelGGL(k_composite_surrogate, dim3((unsigned)(groups < 1024 ? groups : 1024)),
+    hipLaunchKernelGGL(k_composite_surrogate,
+                       dim3((unsigned)(groups < ATMO_SURROGATE_GRID ? groups : ATMO_SURROGATE_GRID)),
                        dim3(256), 0, s, h->k, a, h->queue, ca.parity);
     if (atmo_launch_status(h, "atmo_composite_surrogate") != ATMO_OK) return ATMO_EHIP;
     if (atmo_launch_composite_list(h, ca, s) != ATMO_OK) return ATMO_EHIP;

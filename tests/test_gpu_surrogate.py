@@ -60,7 +60,7 @@ def _random_params(seed, spread=0.05):
 
 
 def test_mfma_inference_matches_quantised_fp64(sur, device):
-    """v_mfma_f32_16x16x16bf16 lane layout and the fused 4-layer chain: the
+    """v_mfma_f32_16x16x32_bf16 lane layout and the fused 4-layer chain: the
     BF16 inference equals a float64 forward with the same quantisation."""
     net = sur.SurrogateMLP(max_batch=4096, device=device)
     params = _random_params(1)
@@ -293,4 +293,86 @@ def test_surrogate_gain_against_fewer_integrator_steps(pkg, scene, sur, ref64, d
     for name, (ms, p99) in results.items():
         print(f"gain: {name:20s} composite {ms * 1e3:8.1f} us  p99 vs f64 {p99:.2e}", flush=True)
     assert results["surrogate"][0] < results["plain 12 intervals"][0]
+    net.close()
+
+
+def folded_quantized_forward(params, x):
+    """quantized_forward with layer 1's altitude and sun-cosine inputs
+    (constant over a frame) folded into its bias, as the surrogate composite
+    evaluates it."""
+    layers = mlp_fp64.unpack(params)
+    w1, b1 = layers[0]
+    x = np.asarray(x, dtype=np.float64)
+    bias0 = (b1 + w1[:, 0] * x[0, 0]) + w1[:, 2] * x[0, 2]
+    a = bf16(mlp_fp64.silu(x[:, [1, 3, 4]] @ w1[:, [1, 3, 4]].T + bias0))
+    for w, b in layers[1:3]:
+        a = bf16(mlp_fp64.silu(a @ bf16(w).T + b))
+    w, b = layers[3]
+    return a @ bf16(w).T + b
+
+
+def test_surrogate_composite_equals_network_and_exact_transmittance(pkg, scene, sur, device):
+    """In-domain surface pixels of the surrogate composite: with a black
+    scene the pixel is E L, where L decodes the network's output on the
+    pixel's encoded inputs (the 16x16x32 BF16 MFMA chain, layer 1 with the
+    frame constants folded into its bias) -- checked against the float64
+    forward with the same quantisation; with the scene, the added term is
+    scene x T with T the plain path's two-sample transmittance
+    (AERIAL_TRANSMITTANCE view)."""
+    import torch
+
+    net = sur.SurrogateMLP.load(sur.default_weights_path(), max_batch=1024, device=device)
+    inputs = scene.make_frame(width=320, height=180, config_index=1, seed_offset=7)
+    light = inputs.lighting
+    r = pkg.AtmosphereRenderer(inputs.width, inputs.height, inputs.config, device)
+    pos, sid = _dev(inputs.position_view, device), _dev(inputs.surface_id, device, np.int32)
+    scene_hdr, space = _dev(inputs.scene_hdr, device), _dev(inputs.space_hdr, device)
+    R, r_cam = inputs.planet_radius_m, inputs.camera_radius_m
+    r.update(R, r_cam, inputs.sun_local, light.solar_irradiance, light.sun_angular_radius_degrees)
+
+    def composite(scene_in, view=0, surrogate=None):
+        out = torch.full_like(scene_hdr, float("nan"))
+        r.composite(scene_in, space, sid, pos, out, R, r_cam, inputs.sun_local,
+                    light.solar_irradiance, light.sun_disk_radiance,
+                    light.sun_angular_radius_degrees, inputs.view_basis, inputs.tangent_half_fov,
+                    *inputs.star_contrast, diagnostic_view=view, surrogate=surrogate)
+        torch.cuda.synchronize(device)
+        return out.cpu().numpy()
+
+    black = composite(torch.zeros_like(scene_hdr), surrogate=net)
+    lit = composite(scene_hdr, surrogate=net)
+    trans = composite(scene_hdr, view=6)
+    surf = inputs.surface_id >= 0
+    h = r_cam - R
+    rays = scene.pixel_rays_view(inputs.width, inputs.height, inputs.tangent_half_fov)
+    right, up, fwd = (np.asarray(v) for v in inputs.view_basis)
+    world = rays[..., :1] * right + rays[..., 1:2] * up + rays[..., 2:3] * fwd
+    world /= np.linalg.norm(world, axis=-1, keepdims=True)
+    sun = np.asarray(inputs.sun_local, dtype=np.float64)
+    d = np.linalg.norm(inputs.position_view.astype(np.float64), axis=-1)
+    vl = np.hypot(world[..., 0], world[..., 2])
+    phi = np.arccos(np.clip((world[..., 0] * sun[0] + world[..., 2] * sun[2])
+                            / (vl * math.hypot(sun[0], sun[2])), -1, 1))
+    raw = np.stack([np.full(surf.sum(), h), world[..., 1][surf], np.full(surf.sum(), sun[1]),
+                    phi[surf], d[surf]], 1)
+    x = sur.encode_host(raw, net.planet_radius_m, net.cosine_stretch)
+    y = folded_quantized_forward(net.get_params().astype(np.float64), x)
+    L, _ = sur.decode_outputs(y, net.out_mean, net.out_scale, net.output_mode)
+    want = L * np.asarray(light.solar_irradiance)
+    got = black[surf]
+    lit_share = want > 1e-3 * want.max()
+    log_err = np.abs(np.log(got[lit_share]) - np.log(want[lit_share]))
+    p99, p999 = np.percentile(log_err, 99), np.percentile(log_err, 99.9)
+    print(f"surrogate in-scatter vs quantised fp64 network: |dlog| max {log_err.max():.2e}, "
+          f"p99.9 {p999:.2e}, p99 {p99:.2e} over {int(lit_share.sum())} values", flush=True)
+    # The device encodes in float32 (acos of the horizontal dot product, asinh
+    # of the horizon-relative cosine), the host in float64: where the network
+    # is steep in its inputs (azimuth near 0, view near the horizon) a few
+    # ulps of a feature move log L by a few 1e-2.  A wrong lane layout or
+    # bias fold moves every pixel by O(1).
+    assert p99 < 3e-3 and p999 < 3e-2 and log_err.max() < 0.2
+    added = lit[surf] - black[surf]
+    t_want = inputs.scene_hdr[surf] * trans[surf]
+    assert np.abs(added - t_want).max() <= 1e-5 * max(np.abs(lit[surf]).max(), 1.0)
+    r.close()
     net.close()
