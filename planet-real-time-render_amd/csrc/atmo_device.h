@@ -1259,6 +1259,34 @@ __device__ __forceinline__ int compact_block(bool flag, int* list, int* wave_cou
     return total;
 }
 
+// Two-sided compaction of a 256-thread block: the threads flagged `a` go to
+// list[0..na) and those flagged `b` (a and b exclusive) to list[na..na+nb),
+// each in thread order; returns na and sets nb.  Every thread of the block
+// must call it; `wave_counts` holds 8 ints.
+__device__ __forceinline__ int compact_block2(bool a, bool b, int* list, int* wave_counts,
+                                              int& nb) {
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const unsigned long long ma = __ballot(a), mb = __ballot(b);
+    if (lane == 0) {
+        wave_counts[wave] = __popcll(ma);
+        wave_counts[4 + wave] = __popcll(mb);
+    }
+    __syncthreads();
+    int offset_a = 0, total_a = 0, offset_b = 0, total_b = 0;
+    for (int w = 0; w < 4; ++w) {
+        offset_a += w < wave ? wave_counts[w] : 0;
+        total_a += wave_counts[w];
+        offset_b += w < wave ? wave_counts[4 + w] : 0;
+        total_b += wave_counts[4 + w];
+    }
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (a) list[offset_a + __popcll(ma & below)] = threadIdx.x;
+    if (b) list[total_a + offset_b + __popcll(mb & below)] = threadIdx.x;
+    __syncthreads();
+    nb = total_b;
+    return total_a;
+}
+
 // Appends the flagged lanes' pixel indices to the deferred-pixel queue (one
 // atomic per wave, by the lowest flagged lane).  Every lane of the wave
 // must call it.  `capacity` is the number of index slots after the two

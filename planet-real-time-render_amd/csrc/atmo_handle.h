@@ -88,10 +88,8 @@ struct BatchState {
     float thf;
 };
 
-// Launch the sky pass of the COMPOSITE view, and the full surface branch
-// for the pixels deferred to a.queue, in the handle's precision variant
-// (atmo_kernels.hip).
-int32_t atmo_launch_composite_sky(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream);
+// Launch the full surface branch of the COMPOSITE view for the pixels
+// deferred to a.queue, in the handle's precision variant (atmo_kernels.hip).
 int32_t atmo_launch_composite_list(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream);
 
 // Clears both counters of a deferred-pixel queue with a one-wave kernel

@@ -437,10 +437,6 @@ void composite_frame_constants(const atmo::Consts& k, CompositeArgs& a) {
     a.sky_horizon = k.pi - (float)acos((double)fminf(1.0f, fmaxf(0.0f, tc)));
 }
 
-int32_t atmo_launch_composite_sky(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream) {
-    ATMO_STAGE(h, launch_composite_sky, h->k, a, stream);
-    return atmo_launch_status(h, "k_composite_sky");
-}
 
 int32_t atmo_launch_composite_list(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream) {
     ATMO_STAGE(h, launch_composite_list, h->k, a, stream);

@@ -23,7 +23,6 @@
                         const int32_t* surface_id, float bottom, float cam_alt, f3 sun,          \
                         float ar, f3 right, f3 up, f3 forward, const float* tlut,                \
                         const float* mslut, float* sun_t, float* sky_l, hipStream_t s);          \
-    void launch_composite_sky(const Consts& k, const CompositeArgs& a, hipStream_t s);          \
     void launch_composite(const Consts& k, const CompositeArgs& a, hipStream_t s);              \
     void launch_composite_list(const Consts& k, const CompositeArgs& a, hipStream_t s);
 

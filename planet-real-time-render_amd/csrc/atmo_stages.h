@@ -553,8 +553,9 @@ __device__ __forceinline__ void composite_sky_pixel(const Consts& k, const Compo
 }
 
 // The COMPOSITE view of _composite (renderer.py:1860-1940), one block per
-// 256 pixels.  Sky pixels are finished by their own thread; surface pixels
-// are then compacted onto the leading threads so waves run full, and run
+// 256 pixels.  Surface pixels are compacted onto the leading threads and sky
+// pixels onto the threads after them, so waves run full (the sky branch in
+// at most two waves of a block instead of all four); surface threads run
 // the coarse-only limb integral.  A surface pixel with an interval whose
 // terminator-refinement weight is positive is deferred to k_composite_list,
 // which keeps the refinement's registers out of this kernel; every pixel is
@@ -563,14 +564,16 @@ __device__ __forceinline__ void composite_sky_pixel(const Consts& k, const Compo
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ATMO_COMPOSITE_WAVES, 8)))
 k_composite(Consts k, CompositeArgs a) {
     __shared__ int list[256];
-    __shared__ int wave_counts[4];
+    __shared__ int wave_counts[8];
     const long npix = (long)a.width * a.height;
     const long base = (long)blockIdx.x * 256;
     const long i = base + threadIdx.x;
     const bool live = i < npix;
     const bool surface = live && a.sid[i] >= 0;
-    if (live && !surface) composite_sky_pixel(k, a, i);
-    const int count = compact_block(surface, list, wave_counts);
+    int nsky;
+    const int count = compact_block2(surface, live && !surface, list, wave_counts, nsky);
+    if ((int)threadIdx.x >= count && (int)threadIdx.x < count + nsky)
+        composite_sky_pixel(k, a, base + list[threadIdx.x]);
     bool refine = false;
     long j = 0;
     if ((int)threadIdx.x < count) {
@@ -587,13 +590,6 @@ k_composite(Consts k, CompositeArgs a) {
         }
     }
     enqueue_wave(refine, j, a.queue, a.parity, a.queue_capacity);
-}
-
-// Sky pixels only: the sky pass of the surrogate composite.
-__global__ void __launch_bounds__(256) k_composite_sky(Consts k, CompositeArgs a) {
-    const long npix = (long)a.width * a.height;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < npix && a.sid[i] < 0) composite_sky_pixel(k, a, i);
 }
 
 // The surface pixels deferred to the queue: the full surface branch of the
@@ -738,11 +734,6 @@ void launch_prepare(const Consts& k, long npix, const float* position_view,
     hipLaunchKernelGGL(k_prepare, dim3(grid_for(npix)), dim3(256), 0, s, k, npix, position_view,
                        surface_id, bottom, cam_alt, sun, ar, right, up, forward, tlut, mslut, sun_t,
                        sky_l);
-}
-
-void launch_composite_sky(const Consts& k, const CompositeArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_composite_sky, dim3(grid_for((long)a.width * a.height)), dim3(256), 0, s,
-                       k, a);
 }
 
 void launch_composite_list(const Consts& k, const CompositeArgs& a, hipStream_t s) {

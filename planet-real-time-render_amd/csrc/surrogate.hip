@@ -364,6 +364,12 @@ constexpr int kFolded32 = 8;
 #ifndef ATMO_SURROGATE_CHUNKS
 #define ATMO_SURROGATE_CHUNKS 1
 #endif
+// At least four waves per SIMD (128 VGPRs): the fused sky branch would
+// otherwise take the kernel to 134 VGPRs and three waves (measured 198 ->
+// 171 us at 1080p).
+#ifndef ATMO_SURROGATE_WAVES
+#define ATMO_SURROGATE_WAVES __attribute__((amdgpu_waves_per_eu(4, 8)))
+#endif
 // Persistent blocks of the surrogate composite (at most; one per 256-pixel
 // group on small frames).
 #ifndef ATMO_SURROGATE_GRID
@@ -644,8 +650,10 @@ struct SurrogateArgs {
     float ar;
     f3 right, up, forward;
     float thf, c0, c1;
-    float aspect;  // width / height
-    f3 fwd_t;      // forward / thf
+    float aspect;        // width / height
+    f3 fwd_t;            // forward / thf
+    float pixel_filter;  // the sun disk's pixel filter (composite_frame_constants)
+    float sky_horizon;   // sky-view horizon zenith at sky_alt
     const uint16_t* frag;
     const float* frag32;
     const float* bias;
@@ -653,24 +661,25 @@ struct SurrogateArgs {
     AtmoSurrogateTransform t;
 };
 
-// The surface pixels of the COMPOSITE view with the surrogate in place of
-// the surface branch's in-scatter (and, unless exact_transmittance is set,
-// transmittance); sky pixels are left to the plain sky pass.  Persistent
-// blocks: the 18 KiB of bf16 weight fragments are staged in LDS once per
-// block, with layer 1's bias folded with the frame's constant altitude and
-// sun-cosine features, and the block then walks 256-pixel groups.  In a
-// group, surface pixels are compacted onto the leading threads, so the
-// network runs in full waves over surface pixels only (the MFMA chain needs
-// the whole wave; a wave runs it when any lane holds an in-domain pixel).
+// The COMPOSITE view with the surrogate in place of the surface branch's
+// in-scatter (and, unless exact_transmittance is set, transmittance); sky
+// pixels take the plain sky branch.  Persistent blocks: the 18 KiB of bf16
+// weight fragments are staged in LDS once per block, with layer 1's bias
+// folded with the frame's constant altitude and sun-cosine features, and
+// the block then walks 256-pixel groups.  In a group, surface pixels are
+// compacted onto the leading threads and sky pixels onto the threads after
+// them, so the network runs in full waves over surface pixels (the MFMA
+// chain needs the whole wave; a wave runs it when any lane holds an
+// in-domain pixel) and the sky branch in at most two waves.
 // Out-of-domain surface pixels are queued for k_composite_list, which keeps
 // the plain integrator's registers out of this kernel.
-__global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, SurrogateArgs a,
+__global__ void __launch_bounds__(256) ATMO_SURROGATE_WAVES k_composite_surrogate(Consts k, SurrogateArgs a,
                                                              int* __restrict__ queue, int parity) {
     __shared__ bf16x8 frag_s[kFragments * 64];
     __shared__ float bias_s[64 * 3 + 16];
     __shared__ float bias0_s[64];
     __shared__ int list[256];
-    __shared__ int wave_counts[4];
+    __shared__ int wave_counts[8];
     const bf16x8* fg = reinterpret_cast<const bf16x8*>(a.frag);
     for (int e = threadIdx.x; e < kFragments * 64; e += blockDim.x) frag_s[e] = fg[e];
     for (int e = threadIdx.x; e < 64 * 3 + 16; e += blockDim.x) bias_s[e] = a.bias[e];
@@ -693,7 +702,17 @@ __global__ void __launch_bounds__(256) k_composite_surrogate(Consts k, Surrogate
         const long base = g * 256;
         const long i = base + threadIdx.x;
         const bool surface = i < npix && a.sid[i] >= 0;
-        const int count = compact_block(surface, list, wave_counts);
+        int nsky;
+        const int count = compact_block2(surface, i < npix && !surface, list, wave_counts, nsky);
+        if ((int)threadIdx.x >= count && (int)threadIdx.x < count + nsky) {
+            const long p = base + list[threadIdx.x];
+            const f3 ray = camera_ray((int)(p / a.height), (int)(p % a.height), a.width, a.height,
+                                      a.aspect, a.right, a.up, a.fwd_t);
+            const f3 c = composite_sky(k, a.tlut, a.svlut, ray, ld3(a.scene, p), ld3(a.space, p),
+                                       a.cam_alt, a.sky_alt, a.sun, a.disk, a.ar, a.pixel_filter,
+                                       a.sky_horizon, a.c0, a.c1, a.bottom);
+            st3(a.hdr, p, maxs(c, 0.0f));
+        }
         const bool has = (int)threadIdx.x < count;
         const long j = base + (has ? list[threadIdx.x] : 0);
         const f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height,
@@ -1071,9 +1090,10 @@ int32_t atmo_composite_surrogate(AtmoHandle* h, AtmoMlp* m, const float* scene_h
     composite_frame_constants(h->k, ca);
     a.aspect = ca.aspect;
     a.fwd_t = ca.fwd_t;
+    a.pixel_filter = ca.pixel_filter;
+    a.sky_horizon = ca.sky_horizon;
     if (atmo_begin_queue(h, ca, s) != ATMO_OK) return ATMO_EHIP;
     h->queue_reset = true;  // until every launch of this call has gone out
-    if (atmo_launch_composite_sky(h, ca, s) != ATMO_OK) return ATMO_EHIP;
     hipLaunchKernelGGL(k_composite_surrogate,
                        dim3((unsigned)(groups < ATMO_SURROGATE_GRID ? groups : ATMO_SURROGATE_GRID)),
                        dim3(256), 0, s, h->k, a, h->queue, ca.parity);
