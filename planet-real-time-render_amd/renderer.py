@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import warnings
 
 import numpy as np
 
@@ -68,6 +69,20 @@ def _device_tensor(value, name: str, shape: tuple[int, ...], dtype, device):
 
 
 PRECISIONS = {"reference": _capi.PRECISION_REFERENCE, "robust": _capi.PRECISION_ROBUST}
+
+
+def _warn_if_below_gate(surrogate) -> None:
+    """One warning per network whose weight file records that it misses the
+    1 % p99 accuracy gate (the surrogate composite then trades accuracy for
+    speed; the plain composite is the accurate path)."""
+    if getattr(surrogate, "gate_met", None) is False and not getattr(surrogate, "_gate_warned", False):
+        validation = surrogate.metadata.get("validation", {})
+        warnings.warn(
+            f"surrogate weights miss the 1% p99 accuracy gate: hdr p99 relative error "
+            f"{validation.get('hdr_p99_vs_fp64')} against the float64 restatement on "
+            f"{validation.get('frame')}; the plain composite is the accurate path",
+            RuntimeWarning, stacklevel=3)
+        surrogate._gate_warned = True
 
 
 class AtmosphereRenderer:
@@ -369,6 +384,7 @@ class AtmosphereRenderer:
         if self.sky_snapshot_altitude_m is not None:
             sky_lut_camera_altitude = self.sky_snapshot_altitude_m
         if surrogate is not None and int(diagnostic_view) == int(AtmosphereDiagnosticView.COMPOSITE):
+            _warn_if_below_gate(surrogate)
             self._composite_surrogate(surrogate, scene_hdr, space_hdr, surface_id, position_view,
                                       hdr, float(planet_radius_m), camera_altitude,
                                       sky_lut_camera_altitude, sun_local, solar_irradiance,

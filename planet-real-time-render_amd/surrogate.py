@@ -389,6 +389,14 @@ class SurrogateMLP:
         })
         path.with_suffix(".json").write_text(json.dumps(meta, indent=1))
 
+    @property
+    def gate_met(self) -> bool | None:
+        """Whether the weight file records the network as meeting the 1 % p99
+        accuracy gate against the float64 restatement on its validation
+        frame (None when the file records no validation)."""
+        validation = self.metadata.get("validation")
+        return None if validation is None else bool(validation.get("gate_met"))
+
     @classmethod
     def load(cls, path, max_batch: int = 65536, device=None) -> "SurrogateMLP":
         path = Path(path)

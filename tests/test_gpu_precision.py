@@ -83,3 +83,65 @@ def test_robust_matches_reference_oracle_where_float32_is_well_conditioned(pkg, 
               f"max {worst:.2e}", flush=True)
         assert share > 0.4, key  # 45-100 % of each array on these frames
         assert worst <= PLAIN_TOLERANCE, key
+
+
+# Upper bounds of the shipped network's hdr p99 against float64 on the
+# in-domain stress frames, about 1.5x the values measured on MI355X (3.56,
+# 2.84, 8.8e-2 at 480x270): it misses the 1 % gate there by up to two orders
+# of magnitude, and a regression fails here.
+STRESS_SURROGATE_P99 = {0.5: 5.5, 2_000.0: 4.3, 100_000.0: 0.13}
+
+
+@pytest.mark.parametrize("altitude", [0.5, 2_000.0, 100_000.0, 400_000.0, 2_000_000.0])
+def test_stress_surrogate_error_vs_fp64(pkg, scene, ref64, device, altitude):
+    """The surrogate leg of the precision stress frames: the shipped network
+    in the composite at each altitude of the sweep, its hdr error against
+    the float64 restatement reported beside the plain path's.  Cameras above
+    the network's 100 km training domain fall back to the plain integrator
+    for every pixel, so there the surrogate frame is the plain frame bit for
+    bit; inside the domain its error is bounded by the values measured on
+    these frames (the network misses the 1 % gate)."""
+    import importlib
+
+    import torch
+
+    sur = importlib.import_module(pkg.__name__ + ".surrogate")
+    net = sur.SurrogateMLP.load(sur.default_weights_path(), max_batch=1024, device=device)
+    inputs = scene.make_stress_frame(altitude, width=480, height=270)
+    want64 = ref64.frame(inputs)["hdr"]
+    light = inputs.lighting
+    r = pkg.AtmosphereRenderer(inputs.width, inputs.height, inputs.config, device)
+    to = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
+    pos, sid, sc, sp = (to(inputs.position_view), to(inputs.surface_id), to(inputs.scene_hdr),
+                        to(inputs.space_hdr))
+    R, rc = inputs.planet_radius_m, inputs.camera_radius_m
+    r.update(R, rc, inputs.sun_local, light.solar_irradiance, light.sun_angular_radius_degrees)
+    outs = {}
+    for name, s in (("plain", None), ("surrogate", net)):
+        hdr = torch.zeros_like(sc)
+        with pytest.warns(RuntimeWarning) if s is not None else _nullcontext():
+            r.composite(sc, sp, sid, pos, hdr, R, rc, inputs.sun_local, light.solar_irradiance,
+                        light.sun_disk_radiance, light.sun_angular_radius_degrees,
+                        inputs.view_basis, inputs.tangent_half_fov, *inputs.star_contrast,
+                        surrogate=s)
+        torch.cuda.synchronize(device)
+        outs[name] = hdr.cpu().numpy()
+    e_plain = summary(rel_error(outs["plain"], want64))
+    e_sur = summary(rel_error(outs["surrogate"], want64))
+    print(f"stress h {altitude} m: hdr vs f64 p99 / max  plain {e_plain['p99']:.2e} / "
+          f"{e_plain['max']:.2e}  surrogate {e_sur['p99']:.2e} / {e_sur['max']:.2e}", flush=True)
+    if altitude > 100_000.0:
+        assert (outs["plain"].view(np.uint32) == outs["surrogate"].view(np.uint32)).all()
+    else:
+        assert np.isfinite(outs["surrogate"]).all()
+        assert e_sur["p99"] <= STRESS_SURROGATE_P99[altitude]
+    r.close()
+    net.close()
+
+
+class _nullcontext:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False

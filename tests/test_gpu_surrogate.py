@@ -244,8 +244,12 @@ def test_surrogate_gain_against_fewer_integrator_steps(pkg, scene, sur, ref64, d
     surrogate against the plain integrator at the reference's 12 base
     intervals and at fewer (4, 6, 8), each timed on the GPU and scored by
     its p99 relative error against the float64 restatement (12 intervals).
-    The surrogate must be faster than the 12-interval path; the equal-time
-    accuracy comparison is printed for the record."""
+    Asserted: the surrogate is faster than the 12-interval path; its error
+    stays within the recorded bounds (p99 0.52 over the frame, 0.60 over
+    surface pixels: the 1 % gate is missed, and a regression fails here);
+    and at equal time the reduced-step integrator is the more accurate (the
+    finding the design records: a fewer-interval plain path at most the
+    surrogate's time has the lower p99)."""
     import torch
 
     weights = sur.default_weights_path()
@@ -281,18 +285,25 @@ def test_surrogate_gain_against_fewer_integrator_steps(pkg, scene, sur, ref64, d
             call()
         e1.record()
         torch.cuda.synchronize(device)
-        p99 = summary(rel_error(hdr.cpu().numpy(), want))["p99"]
+        err = rel_error(hdr.cpu().numpy(), want)
         r.close()
-        return e0.elapsed_time(e1) / reps, p99
+        surf = inputs.surface_id >= 0
+        return (e0.elapsed_time(e1) / reps, float(np.percentile(err, 99)),
+                float(np.percentile(err[surf], 99)), float(err[surf].max()))
 
     base = inputs.config
     results = {"surrogate": run(base, net)}
     for steps in (12, 8, 6, 4):
         cfg = base.replace(aerial_horizon_raymarch_steps=steps)
         results[f"plain {steps} intervals"] = run(cfg, None)
-    for name, (ms, p99) in results.items():
-        print(f"gain: {name:20s} composite {ms * 1e3:8.1f} us  p99 vs f64 {p99:.2e}", flush=True)
-    assert results["surrogate"][0] < results["plain 12 intervals"][0]
+    for name, (ms, p99, p99s, worst) in results.items():
+        print(f"gain: {name:20s} composite {ms * 1e3:8.1f} us  p99 vs f64 {p99:.2e} (surface "
+              f"pixels p99 {p99s:.2e}, max {worst:.2e})", flush=True)
+    t_sur, p99_sur, p99s_sur, _ = results["surrogate"]
+    assert t_sur < results["plain 12 intervals"][0]
+    assert p99_sur <= 0.6 and p99s_sur <= 0.7
+    assert any(t <= t_sur and p99 < p99_sur for name, (t, p99, _, _) in results.items()
+               if name.startswith("plain"))
     net.close()
 
 

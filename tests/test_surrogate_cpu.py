@@ -120,3 +120,30 @@ def test_init_params_layout(pkg):
     for (w, b), (fan_in, fan_out) in zip(layers, mlp_fp64.LAYERS):
         assert w.shape == (fan_out, fan_in) and np.all(b == 0.0)
         assert np.abs(w).max() <= math.sqrt(6.0 / fan_in)
+
+
+def test_shipped_weights_record_their_gate_and_the_facade_warns(pkg):
+    """The shipped network's weight file records its validation against the
+    float64 restatement (it misses the 1 % p99 gate), and the facade's
+    composite warns once per network before using such weights."""
+    import importlib
+    import json
+    import warnings
+
+    sur = importlib.import_module(pkg.__name__ + ".surrogate")
+    renderer = importlib.import_module(pkg.__name__ + ".renderer")
+    meta = json.loads(sur.default_weights_path().with_suffix(".json").read_text())
+    validation = meta["validation"]
+    assert validation["gate_p99"] == 0.01
+    assert validation["gate_met"] is (validation["hdr_p99_vs_fp64"] <= 0.01)
+
+    class Net:  # the attributes the facade reads, without a device
+        metadata = meta
+        gate_met = sur.SurrogateMLP.gate_met.fget(type("M", (), {"metadata": meta})())
+
+    net = Net()
+    with warnings.catch_warnings(record=True) as caught:
+        warnings.simplefilter("always")
+        renderer._warn_if_below_gate(net)
+        renderer._warn_if_below_gate(net)
+    assert len(caught) == 1 and "accuracy gate" in str(caught[0].message)
