@@ -14,9 +14,10 @@
 // float64, rounded once), the definition the CPU restatement uses, wherever
 // the reference's formulas amplify last-bit differences: finite-disk
 // visibility and the representative sun zenith near the horizon,
-// (1 - exp(-x)) / s for small x, the sky-view horizon coordinate.  Where
-// nothing downstream amplifies a last-bit difference (density exp, exp of
-// long segments) the hardware-rate function is used; each such site says so.
+// (1 - exp(-x)) / s for small x together with the density exp that feeds
+// x, the sky-view horizon coordinate.  Where nothing downstream amplifies a
+// last-bit difference (exp of long segments) the hardware-rate function is
+// used; each such site says so.
 //
 // Frame- and LUT-sized arrays use the reference's Taichi field layout:
 // (X, Y[, Z], 3) float32, element [x, y] at (x * Y + y) * 3.
@@ -139,6 +140,7 @@ __device__ __forceinline__ double ddiv(double a, double b) {
 }
 
 __device__ __forceinline__ float cr_exp(float x) { return (float)exp((double)x); }
+
 __device__ __forceinline__ float cr_acos(float x) { return (float)acos_d((double)x); }
 __device__ __forceinline__ float cr_cos(float x) { return (float)cos((double)x); }
 __device__ __forceinline__ float cr_sin(float x) { return (float)sin((double)x); }
@@ -276,13 +278,17 @@ inline namespace ATMO_PRECISION {
 __device__ __forceinline__ f3 density(const Consts& k, float h) {
     float rayleigh = 0.0f, mie = 0.0f, absorption = 0.0f;
     if (0.0f <= h && h <= k.top) {
-        // Hardware-rate exp: a last-bit difference in exp moves the
-        // extinction by one ulp.  The exponent h / H must be the correctly
-        // rounded quotient (quot): twilight texels are lit through grazing
-        // solar paths of optical depth ~40, where even the 1-2 ulp of
-        // h * (1 / H) show as 4e-4.
-        rayleigh = expf(-quot(h, k.h_r, k.r_h_r));
-        mie = expf(-quot(h, k.h_m, k.r_h_m));
+        // Correctly rounded exp, as the exponent h / H (quot) is: the
+        // extinction must carry the reference's bits.  One ulp of it moves
+        // x = sigma * step, and for x < 1/16 that can flip the rounding of
+        // the correctly rounded exp(-x) of the segment integral, which
+        // (1 - exp(-x)) / sigma amplifies by 1 / x: a hardware-rate exp here
+        // put a twilight pixel 9.6e-5 from the float32 restatement; this one
+        // 3.8e-6.  h / H must be correctly rounded for the same reason:
+        // twilight texels are lit through grazing solar paths of optical
+        // depth ~40, where even the 1-2 ulp of h * (1 / H) show as 4e-4.
+        rayleigh = cr_exp(-quot(h, k.h_r, k.r_h_r));
+        mie = cr_exp(-quot(h, k.h_m, k.r_h_m));
         absorption = fmaxf(1.0f - quot(fabsf(h - k.peak), k.half_width, k.r_half_width), 0.0f);
     }
     return f3{rayleigh, mie, absorption};
