@@ -117,9 +117,16 @@ typedef struct AtmoHandle AtmoHandle;
 int32_t atmo_validate_params(const AtmoParams* params, char* message,
                              int32_t message_capacity);
 
+/* Largest frame a handle accepts, width x height: 2^28 pixels (16384 x
+ * 16384), so that a pixel's byte offset in a (W, H, 3) float32 field fits
+ * in 32 bits.  The reference sets no limit; its fields at this size would
+ * take 3 GiB each. */
+#define ATMO_MAX_PIXELS 268435456
+
 /* Replaces `AtmosphereRenderer.__init__(width, height, config)`
  * (renderer.py:51-116): validates the configuration and allocates the LUT
- * and surface-lighting buffers on the current HIP device. */
+ * and surface-lighting buffers on the current HIP device.  Frames above
+ * ATMO_MAX_PIXELS return ATMO_EINVAL. */
 int32_t atmo_create(const AtmoParams* params, int32_t width, int32_t height,
                     AtmoHandle** out);
 int32_t atmo_destroy(AtmoHandle* handle);

@@ -29,6 +29,7 @@ BUF_SURFACE_SKY_L = 5
 PRECISION_REFERENCE = 0
 PRECISION_ROBUST = 1
 
+MAX_PIXELS = 268435456  # ATMO_MAX_PIXELS
 NOALLOC_MAX_PIXELS = 8388608  # ATMO_NOALLOC_MAX_PIXELS
 
 # Every symbol include/atmo.h declares, checked by the CPU test suite.

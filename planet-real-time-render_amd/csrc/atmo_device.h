@@ -249,6 +249,19 @@ __device__ __forceinline__ f3 lut3(const float* __restrict__ base, uint32_t off)
     const float* p = reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + off);
     return f3{p[0], p[1], p[2]};
 }
+// Pixel i of a (W, H, 3) float32 frame field by a 32-bit byte offset: a
+// frame holds at most 2^28 pixels (atmo_create rejects larger ones), so
+// 12 i < 2^32 and the access takes the scalar-base, vector-offset form with
+// no 64-bit address arithmetic.
+__device__ __forceinline__ f3 pix3(const float* __restrict__ base, uint32_t i) {
+    return lut3(base, i * 12u);
+}
+__device__ __forceinline__ void put3(float* __restrict__ base, uint32_t i, f3 v) {
+    float* p = reinterpret_cast<float*>(reinterpret_cast<char*>(base) + i * 12u);
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+}
 __device__ __forceinline__ f3 ld3(const float* __restrict__ base, long i) {
     return f3{base[i * 3 + 0], base[i * 3 + 1], base[i * 3 + 2]};
 }
@@ -1308,7 +1321,10 @@ __device__ __forceinline__ void enqueue_wave(bool flag, long pixel, int* __restr
     int slot = 0;
     if (lane == leader) slot = atomicAdd(queue + parity, __popcll(mask));
     slot = __shfl(slot, leader, 64);
-    const long at = (long)slot + __popcll(mask & ((1ull << lane) - 1ull));
+    // flagged lanes below this one: v_mbcnt over the ballot, no 64-bit lane mask
+    const unsigned below = __builtin_amdgcn_mbcnt_hi(
+        (unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+    const long at = (long)slot + below;
     if (flag && at < capacity) queue[2 + at] = (int)pixel;
 }
 

@@ -240,6 +240,13 @@ int32_t atmo_create(const AtmoParams* params, int32_t width, int32_t height, Atm
         set_error(nullptr, "width and height must be positive");
         return ATMO_EINVAL;
     }
+    if ((long)width * (long)height > (long)ATMO_MAX_PIXELS) {
+        char msg[160];
+        snprintf(msg, sizeof msg, "atmo_create: %d x %d pixels exceed the limit of %ld pixels",
+                 (int)width, (int)height, (long)ATMO_MAX_PIXELS);
+        set_error(nullptr, msg);
+        return ATMO_EINVAL;
+    }
     AtmoHandle* h = new (std::nothrow) AtmoHandle();
     if (!h) {
         set_error(nullptr, "atmo_create: host allocation failed");

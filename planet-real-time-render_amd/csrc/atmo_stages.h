@@ -495,13 +495,13 @@ __global__ void __launch_bounds__(256) k_prepare(Consts k, long npix,
                                                  const float* __restrict__ mslut,
                                                  float* __restrict__ sun_t,
                                                  float* __restrict__ sky_l) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;  // npix <= 2^28
     if (i >= npix) return;
     f3 solar = mk(1.0f, 1.0f, 1.0f);
     f3 sky = mk(0.0f, 0.0f, 0.0f);
     if (surface_id[i] >= 0) {
         f3 camera = mk(0.0f, bottom + cam_alt, 0.0f);
-        f3 vp = ld3(position_view, i);
+        f3 vp = pix3(position_view, i);
         f3 local = (right * vp.x + up * vp.y) + forward * vp.z;
         f3 planet = camera + local;
         f3 radial = div3(planet, fmaxf(norm(planet), 1.0f));
@@ -528,8 +528,8 @@ __global__ void __launch_bounds__(256) k_prepare(Consts k, long npix,
         solar = solar_transmittance(k, tlut, g, h, bottom, ar);
         sky = sample_multi_scattering(k, mslut, h, sc, bottom);
     }
-    st3(sun_t, i, solar);
-    st3(sky_l, i, maxs(sky, 0.0f));
+    put3(sun_t, i, solar);
+    put3(sky_l, i, maxs(sky, 0.0f));
 }
 
 
@@ -537,19 +537,20 @@ __device__ __forceinline__ f3 pixel_ray(const CompositeArgs& a, int px, int py, 
                                         float& sv) {
     return camera_ray(px, py, a.width, a.height, a.aspect, a.right, a.up, a.fwd_t, su, sv);
 }
-__device__ __forceinline__ f3 pixel_ray(const CompositeArgs& a, long i) {
+__device__ __forceinline__ f3 pixel_ray(const CompositeArgs& a, uint32_t i) {
     float su, sv;
-    return pixel_ray(a, (int)(i / a.height), (int)(i % a.height), su, sv);
+    const uint32_t px = i / (uint32_t)a.height;
+    return pixel_ray(a, (int)px, (int)(i - px * (uint32_t)a.height), su, sv);
 }
 
 // Sky branch of one COMPOSITE pixel (renderer.py:1882-1940).
 __device__ __forceinline__ void composite_sky_pixel(const Consts& k, const CompositeArgs& a,
-                                                    long i) {
+                                                    uint32_t i) {
     f3 ray = pixel_ray(a, i);
-    f3 c = composite_sky(k, a.tlut, a.svlut, ray, ld3(a.scene, i), ld3(a.space, i), a.cam_alt,
+    f3 c = composite_sky(k, a.tlut, a.svlut, ray, pix3(a.scene, i), pix3(a.space, i), a.cam_alt,
                          a.sky_alt, a.sun, a.disk, a.ar, a.pixel_filter, a.sky_horizon, a.c0, a.c1,
                          a.bottom);
-    st3(a.hdr, i, maxs(c, 0.0f));
+    put3(a.hdr, i, maxs(c, 0.0f));
 }
 
 // The COMPOSITE view of _composite (renderer.py:1860-1940), one block per
@@ -565,9 +566,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ATMO_C
 k_composite(Consts k, CompositeArgs a) {
     __shared__ int list[256];
     __shared__ int wave_counts[8];
-    const long npix = (long)a.width * a.height;
-    const long base = (long)blockIdx.x * 256;
-    const long i = base + threadIdx.x;
+    __shared__ float4 stash[256];
+    const uint32_t npix = (uint32_t)a.width * (uint32_t)a.height;
+    const uint32_t base = blockIdx.x * 256u;
+    const uint32_t i = base + threadIdx.x;
     const bool live = i < npix;
     const bool surface = live && a.sid[i] >= 0;
     int nsky;
@@ -575,16 +577,37 @@ k_composite(Consts k, CompositeArgs a) {
     if ((int)threadIdx.x >= count && (int)threadIdx.x < count + nsky)
         composite_sky_pixel(k, a, base + list[threadIdx.x]);
     bool refine = false;
-    long j = 0;
+    uint32_t j = 0;
+    // the wave's first thread index, wave-uniform (an SGPR)
+    const uint32_t wave_base = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
     if ((int)threadIdx.x < count) {
-        j = base + list[threadIdx.x];
-        f3 ray = pixel_ray(a, j);
-        float sd = norm(ld3(a.pos, j));
+        {
+            const uint32_t p = base + list[threadIdx.x];
+            const f3 ray = pixel_ray(a, p);
+            const float sd = norm(pix3(a.pos, p));
+            stash[threadIdx.x] = make_float4(ray.x, ray.y, ray.z, sd);
+        }
         f3 scat;
-        if (surface_inscatter_t<false>(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun, a.irradiance,
-                                       a.bottom, a.ar, scat)) {
-            f3 trans = surface_transmittance(k, a.tlut, a.cam_alt, ray, sd, a.bottom);
-            st3(a.hdr, j, maxs(scat + ld3(a.scene, j) * trans, 0.0f));
+        bool done;
+        {
+            const float4 r = stash[threadIdx.x];
+            done = surface_inscatter_t<false>(k, a.tlut, a.mslut, a.cam_alt, mk(r.x, r.y, r.z), r.w,
+                                              a.sun, a.irradiance, a.bottom, a.ar, scat);
+        }
+        // The pixel, its ray and its distance come back from LDS after the
+        // interval loop instead of staying live in registers across it (at
+        // four waves per SIMD they would spill to scratch, i.e. to HBM).
+        // The empty asm stops the compiler from forwarding the values, and
+        // the thread index is rebuilt from the wave base and v_mbcnt so that
+        // not even the LDS address stays live.
+        asm volatile("" ::: "memory");
+        const uint32_t t = wave_base + __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+        j = base + list[t];
+        const float4 r = stash[t];
+        const f3 ray = mk(r.x, r.y, r.z);
+        if (done) {
+            f3 trans = surface_transmittance(k, a.tlut, a.cam_alt, ray, r.w, a.bottom);
+            put3(a.hdr, j, maxs(scat + pix3(a.scene, j) * trans, 0.0f));
         } else {
             refine = true;
         }
@@ -600,22 +623,22 @@ __global__ void __launch_bounds__(256) k_composite_list(Consts k, CompositeArgs 
     const int count = (int)min((long)a.queue[a.parity], a.queue_capacity);
     if (blockIdx.x == 0 && threadIdx.x == 0) a.queue[1 - a.parity] = 0;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < count; e += gridDim.x * blockDim.x) {
-        const long j = a.queue[2 + e];
+        const uint32_t j = (uint32_t)a.queue[2 + e];
         f3 ray = pixel_ray(a, j);
-        float sd = norm(ld3(a.pos, j));
+        float sd = norm(pix3(a.pos, j));
         f3 scat = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun, a.irradiance,
                                     a.bottom, a.ar);
         f3 trans = surface_transmittance(k, a.tlut, a.cam_alt, ray, sd, a.bottom);
-        st3(a.hdr, j, maxs(scat + ld3(a.scene, j) * trans, 0.0f));
+        put3(a.hdr, j, maxs(scat + pix3(a.scene, j) * trans, 0.0f));
     }
 }
 
 // Diagnostic views 1..10 of _composite (renderer.py:1941-2055).
 __global__ void __launch_bounds__(256) k_composite_diagnostic(Consts k, CompositeArgs a) {
-    const long npix = (long)a.width * a.height;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = (uint32_t)a.width * (uint32_t)a.height;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npix) return;
-    const int px = (int)(i / a.height), py = (int)(i % a.height);
+    const int px = (int)(i / (uint32_t)a.height), py = (int)(i % (uint32_t)a.height);
     float su, sv;
     f3 ray = pixel_ray(a, px, py, su, sv);
     const bool surface = a.sid[i] >= 0;
@@ -642,12 +665,12 @@ __global__ void __launch_bounds__(256) k_composite_diagnostic(Consts k, Composit
         case ATMO_VIEW_AERIAL_SCATTERING:
         case ATMO_VIEW_AERIAL_DIRECT_ONLY:
             if (surface)
-                color = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, norm(ld3(a.pos, i)),
+                color = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, norm(pix3(a.pos, i)),
                                           a.sun, a.irradiance, a.bottom, a.ar);
             break;
         case ATMO_VIEW_AERIAL_FROXEL_ONLY:
             if (surface)
-                color = sample_aerial(k, a.aplut, su, sv, norm(ld3(a.pos, i)), a.cam_alt, ray,
+                color = sample_aerial(k, a.aplut, su, sv, norm(pix3(a.pos, i)), a.cam_alt, ray,
                                       a.bottom);
             break;
         case ATMO_VIEW_AERIAL_BLEND_WEIGHT: {
@@ -657,7 +680,7 @@ __global__ void __launch_bounds__(256) k_composite_diagnostic(Consts k, Composit
         }
         case ATMO_VIEW_AERIAL_TRANSMITTANCE:
             if (surface)
-                color = surface_transmittance(k, a.tlut, a.cam_alt, ray, norm(ld3(a.pos, i)),
+                color = surface_transmittance(k, a.tlut, a.cam_alt, ray, norm(pix3(a.pos, i)),
                                               a.bottom);
             break;
         default: {
@@ -666,7 +689,7 @@ __global__ void __launch_bounds__(256) k_composite_diagnostic(Consts k, Composit
             break;
         }
     }
-    st3(a.hdr, i, maxs(color, 0.0f));
+    put3(a.hdr, i, maxs(color, 0.0f));
 }
 
 

@@ -696,28 +696,32 @@ __global__ void __launch_bounds__(256) ATMO_SURROGATE_WAVES k_composite_surrogat
 #pragma unroll
     for (int f = 0; f < 4; ++f) a32f[f] = a.frag32[(kFolded32 + f) * 64 + lane];
 
-    const long npix = (long)a.width * a.height;
-    const long groups = (npix + 255) / 256;
-    for (long g = blockIdx.x; g < groups; g += gridDim.x) {
-        const long base = g * 256;
-        const long i = base + threadIdx.x;
+    // pixel byte offsets in 32 bits: a frame holds at most 2^28 pixels
+    const uint32_t npix = (uint32_t)a.width * (uint32_t)a.height;
+    const uint32_t height = (uint32_t)a.height;
+    const uint32_t groups = (npix + 255u) / 256u;
+    for (uint32_t g = blockIdx.x; g < groups; g += gridDim.x) {
+        const uint32_t base = g * 256u;
+        const uint32_t i = base + threadIdx.x;
         const bool surface = i < npix && a.sid[i] >= 0;
         int nsky;
         const int count = compact_block2(surface, i < npix && !surface, list, wave_counts, nsky);
         if ((int)threadIdx.x >= count && (int)threadIdx.x < count + nsky) {
-            const long p = base + list[threadIdx.x];
-            const f3 ray = camera_ray((int)(p / a.height), (int)(p % a.height), a.width, a.height,
+            const uint32_t p = base + list[threadIdx.x];
+            const uint32_t px = p / height;
+            const f3 ray = camera_ray((int)px, (int)(p - px * height), a.width, a.height,
                                       a.aspect, a.right, a.up, a.fwd_t);
-            const f3 c = composite_sky(k, a.tlut, a.svlut, ray, ld3(a.scene, p), ld3(a.space, p),
+            const f3 c = composite_sky(k, a.tlut, a.svlut, ray, pix3(a.scene, p), pix3(a.space, p),
                                        a.cam_alt, a.sky_alt, a.sun, a.disk, a.ar, a.pixel_filter,
                                        a.sky_horizon, a.c0, a.c1, a.bottom);
-            st3(a.hdr, p, maxs(c, 0.0f));
+            put3(a.hdr, p, maxs(c, 0.0f));
         }
         const bool has = (int)threadIdx.x < count;
-        const long j = base + (has ? list[threadIdx.x] : 0);
-        const f3 ray = camera_ray((int)(j / a.height), (int)(j % a.height), a.width, a.height,
+        const uint32_t j = base + (has ? list[threadIdx.x] : 0);
+        const uint32_t jx = j / height;
+        const f3 ray = camera_ray((int)jx, (int)(j - jx * height), a.width, a.height,
                                   a.aspect, a.right, a.up, a.fwd_t);
-        const float sd = has ? norm(ld3(a.pos, j)) : 1.0f;
+        const float sd = has ? norm(pix3(a.pos, j)) : 1.0f;
         const bool learned = has && in_domain(a.t, a.cam_alt, sd);
         // queue the out-of-domain surface pixels
         enqueue_wave(has && !learned, j, queue, parity, npix);
@@ -738,7 +742,7 @@ __global__ void __launch_bounds__(256) ATMO_SURROGATE_WAVES k_composite_surrogat
                 decode(a.t, raw, L, T);
                 if (a.t.exact_transmittance)
                     T = surface_transmittance(k, a.tlut, a.cam_alt, ray, sd, a.bottom);
-                st3(a.hdr, j, maxs(L * a.irradiance + ld3(a.scene, j) * T, 0.0f));
+                put3(a.hdr, j, maxs(L * a.irradiance + pix3(a.scene, j) * T, 0.0f));
             }
         }
         __syncthreads();  // `list` is rewritten by the next group
