@@ -543,14 +543,42 @@ __device__ __forceinline__ f3 pixel_ray(const CompositeArgs& a, uint32_t i) {
     return pixel_ray(a, (int)px, (int)(i - px * (uint32_t)a.height), su, sv);
 }
 
-// Sky branch of one COMPOSITE pixel (renderer.py:1882-1940).
-__device__ __forceinline__ void composite_sky_pixel(const Consts& k, const CompositeArgs& a,
-                                                    uint32_t i) {
+// Sky branch of one COMPOSITE pixel (renderer.py:1882-1940), clamped.
+__device__ __forceinline__ f3 composite_sky_pixel(const Consts& k, const CompositeArgs& a,
+                                                  uint32_t i) {
     f3 ray = pixel_ray(a, i);
     f3 c = composite_sky(k, a.tlut, a.svlut, ray, pix3(a.scene, i), pix3(a.space, i), a.cam_alt,
                          a.sky_alt, a.sun, a.disk, a.ar, a.pixel_filter, a.sky_horizon, a.c0, a.c1,
                          a.bottom);
-    put3(a.hdr, i, maxs(c, 0.0f));
+    return maxs(c, 0.0f);
+}
+
+// The block's 256 output pixels leave through an LDS tile, written to HBM
+// by whichever of the block's four waves finishes last (an LDS counter
+// decides).  Sky and surface pixels share cache lines, and their threads
+// finish tens of microseconds apart: written directly, a line evicted
+// between the two writes was written back twice.  Waves that finish early
+// still exit at once: there is no block-wide barrier.
+__device__ __forceinline__ void tile_put(float* tile, uint32_t l, f3 v) {
+    tile[3 * l + 0] = v.x;
+    tile[3 * l + 1] = v.y;
+    tile[3 * l + 2] = v.z;
+}
+__device__ __forceinline__ void store_tile_by_last_wave(const float* tile, int* waves_done,
+                                                        float* __restrict__ hdr, uint32_t base,
+                                                        uint32_t npix) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // this wave's tile writes
+    const int lane = threadIdx.x % 64;
+    int before = 0;
+    if (lane == 0) before = atomicAdd(waves_done, 1);
+    before = __shfl(before, 0, 64);
+    if (before != 3) return;  // wave-uniform: not the last of the four waves
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // the other waves' tile writes
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t l = lane + 64 * r;
+        if (base + l < npix) put3(hdr, base + l, f3{tile[3 * l], tile[3 * l + 1], tile[3 * l + 2]});
+    }
 }
 
 // The COMPOSITE view of _composite (renderer.py:1860-1940), one block per
@@ -567,15 +595,20 @@ k_composite(Consts k, CompositeArgs a) {
     __shared__ int list[256];
     __shared__ int wave_counts[8];
     __shared__ float4 stash[256];
+    __shared__ float tile[256 * 3];
+    __shared__ int waves_done;
     const uint32_t npix = (uint32_t)a.width * (uint32_t)a.height;
     const uint32_t base = blockIdx.x * 256u;
     const uint32_t i = base + threadIdx.x;
     const bool live = i < npix;
     const bool surface = live && a.sid[i] >= 0;
+    if (threadIdx.x == 0) waves_done = 0;  // ordered by the compaction's barriers
     int nsky;
     const int count = compact_block2(surface, live && !surface, list, wave_counts, nsky);
-    if ((int)threadIdx.x >= count && (int)threadIdx.x < count + nsky)
-        composite_sky_pixel(k, a, base + list[threadIdx.x]);
+    if ((int)threadIdx.x >= count && (int)threadIdx.x < count + nsky) {
+        const uint32_t l = list[threadIdx.x];
+        tile_put(tile, l, composite_sky_pixel(k, a, base + l));
+    }
     bool refine = false;
     uint32_t j = 0;
     // the wave's first thread index, wave-uniform (an SGPR)
@@ -602,17 +635,21 @@ k_composite(Consts k, CompositeArgs a) {
         // not even the LDS address stays live.
         asm volatile("" ::: "memory");
         const uint32_t t = wave_base + __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-        j = base + list[t];
+        const uint32_t l = list[t];
+        j = base + l;
         const float4 r = stash[t];
         const f3 ray = mk(r.x, r.y, r.z);
+        f3 c = mk(0.0f, 0.0f, 0.0f);  // a deferred pixel: k_composite_list writes it later
         if (done) {
             f3 trans = surface_transmittance(k, a.tlut, a.cam_alt, ray, r.w, a.bottom);
-            put3(a.hdr, j, maxs(scat + pix3(a.scene, j) * trans, 0.0f));
+            c = maxs(scat + pix3(a.scene, j) * trans, 0.0f);
         } else {
             refine = true;
         }
+        tile_put(tile, l, c);
     }
     enqueue_wave(refine, j, a.queue, a.parity, a.queue_capacity);
+    store_tile_by_last_wave(tile, &waves_done, a.hdr, base, npix);
 }
 
 // The surface pixels deferred to the queue: the full surface branch of the
