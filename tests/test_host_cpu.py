@@ -131,6 +131,14 @@ def test_create_rejects_invalid_arguments_without_gpu_work(pkg):
     assert not handle.value
     assert lib.atmo_destroy(None) == capi.ATMO_EHANDLE
     assert lib.atmo_build_transmittance(None, 6.36e6, None) == capi.ATMO_EHANDLE
+    # frames above ATMO_MAX_PIXELS (pixel byte offsets must fit in 32 bits)
+    good = pkg.AtmosphereConfig().to_params()
+    assert capi.MAX_PIXELS * 12 < 2**32
+    assert lib.atmo_create(ctypes.byref(good), 16384, 16385, ctypes.byref(handle)) == \
+        capi.ATMO_EINVAL
+    assert b"exceed the limit of 268435456 pixels" in lib.atmo_last_error(None)
+    assert not handle.value
+    assert lib.atmo_create(ctypes.byref(good), 0, 48, ctypes.byref(handle)) == capi.ATMO_EINVAL
 
 
 def test_missing_library_and_device_raise(pkg, monkeypatch, tmp_path):
