@@ -68,7 +68,28 @@ COMPOSITE_BYTES_PER_PIXEL = 4 + 12 + 12 + 12
 # HBM bytes one 1080p k_composite launch moves, from the FETCH_SIZE and
 # WRITE_SIZE counter passes (scaled by a 256 MiB calibration copy) recorded
 # in profiles/atmosphere_frame_1080p_hbm_traffic.md.
-COMPOSITE_TRAFFIC_BYTES_1080P = 133.15e6
+COMPOSITE_TRAFFIC_BYTES_1080P = 108.82e6
+# Memory moves whole cache lines (128 B on MI355X's L2).
+CACHE_LINE_BYTES = 128
+
+
+def composite_line_bytes(surface_id) -> int:
+    """Bytes of the 128-byte lines one COMPOSITE launch must touch: every
+    line of surface_id, scene_hdr and hdr, the lines of position_view that
+    hold a surface pixel and those of space_hdr that hold a sky pixel.  With
+    sky and surface pixels interleaved, most lines of position_view and
+    space_hdr hold both, so this exceeds the per-element bytes."""
+    sid = np.asarray(surface_id).reshape(-1)
+    n = sid.size
+    idx = np.arange(n, dtype=np.int64)
+
+    def lines(sel, elem):
+        i = idx[sel]
+        return np.union1d((i * elem) // CACHE_LINE_BYTES,
+                          (i * elem + elem - 1) // CACHE_LINE_BYTES).size * CACHE_LINE_BYTES
+
+    every = np.ones(n, dtype=bool)
+    return int(lines(every, 4) + 2 * lines(every, 12) + lines(sid >= 0, 12) + lines(sid < 0, 12))
 
 
 def log(*args):
@@ -687,6 +708,7 @@ def main() -> int:
     achieved_tflops = flops / comp_s / 1e12
     p_t = probe_exp_rate()
     bytes_per_launch = COMPOSITE_BYTES_PER_PIXEL * n_pix
+    line_bytes = composite_line_bytes(inputs.surface_id)
     roofline = {
         "bound": "valu",
         "achieved": round(achieved_tflops, 3),
@@ -698,6 +720,9 @@ def main() -> int:
         "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
         "kernel": "k_composite + k_composite_list (atmo_composite, COMPOSITE view)",
         "algorithmic_bytes": bytes_per_launch,
+        "line_bound_bytes": line_bytes,
+        "traffic_vs_line_bound": (round(COMPOSITE_TRAFFIC_BYTES_1080P / line_bytes, 4)
+                                  if (inputs.width, inputs.height) == (1920, 1080) else None),
         "kernel_ms": round(stage_ms["composite"], 4),
         "algorithmic_flop": flops,
         "algorithmic_transcendental": tops,
