@@ -256,6 +256,93 @@ __device__ __forceinline__ void multi_scattering_body(const Consts& k, float bot
     }
 }
 
+// _build_multi_scattering (renderer.py:1065-1206), one wave per texel: lane
+// d marches direction d (then d + 64, ... when there are more than 64) through
+// all of its steps in order, as the reference's loop does, adds the ground
+// bounce, and the wave sums the directions by a butterfly.  For the batched
+// build, whose states and texels give far more waves than the GPU holds, this
+// spends no lanes on chunk scans or on repeating each direction's set-up in
+// every chunk.  The values equal multi_scattering_body's up to float
+// reassociation (the chunk scan and the order of the direction sum).
+__device__ __forceinline__ void multi_scattering_wave(const Consts& k, float bottom, f3 irradiance,
+                                                      float ar, const float* __restrict__ tlut,
+                                                      float* __restrict__ mslut, long texel) {
+    const int lane = threadIdx.x % 64;
+    const int x = (int)(texel / k.mh), y = (int)(texel % k.mh);
+    float u = ((float)x + 0.5f) / (float)k.mw;
+    float v = ((float)y + 0.5f) / (float)k.mh;
+    float h0 = v * v * k.top;
+    float radius = bottom + h0;
+    float sun_c = sun_u_to_cosine(u, h0, bottom);
+    float sun_s = sqrtf(fmaxf(1.0f - sun_c * sun_c, 0.0f));
+    f3 sun = mk(sun_s, sun_c, 0.0f);
+    f3 origin = mk(0.0f, radius, 0.0f);
+    const int N = k.ms_dirs;
+    const int S = k.ms_steps;
+    f3 lum = mk(0.0f, 0.0f, 0.0f), fb = mk(0.0f, 0.0f, 0.0f);
+    for (int base = 0; base < N; base += 64) {
+        const int di = base + lane;
+        if (di >= N) continue;
+        float si = (float)di + 0.5f;
+        float dc = 1.0f - 2.0f * si / (float)N;
+        float ds = sqrtf(fmaxf(1.0f - dc * dc, 0.0f));
+        float az = (float)di * 2.39996323f;
+        float saz, caz;
+        cr_sincos(az, saz, caz);
+        f3 ray = mk(ds * caz, dc, ds * saz);
+        float start, end;
+        bool valid, ground;
+        segment_vec(k, origin, ray, bottom, start, end, valid, ground);
+        if (!valid) continue;
+        f3 dl = mk(0.0f, 0.0f, 0.0f), df = mk(0.0f, 0.0f, 0.0f), pt = mk(1.0f, 1.0f, 1.0f);
+        Warp w = make_warp(-dot(origin, ray), start, end);
+        RayPhase ph = ray_phase(k, ray, sun);
+        const float fS = (float)S, rS = 1.0f / fS;
+        float d0 = boundary(w, 0.0f);
+        for (int s = 0; s < S; ++s) {
+            float d1 = boundary(w, quot((float)(s + 1), fS, rS));
+            float dist = (d0 + d1) * 0.5f;
+            float step = d1 - d0;
+            f3 point = origin + ray * dist;
+            float h = altitude_on_ray(h0, dc, dist, bottom);
+            f3 d = density(k, h);
+            f3 st, si3;
+            segment_integral(extinction_of(k, d), step, st, si3);
+            f3 direct = scattering_source<false>(k, tlut, nullptr, point, h, d, ph, sun, irradiance,
+                                                 bottom, ar);
+            dl = dl + (pt * direct) * si3;
+            df = df + (pt * scattering_of(k, d)) * si3;
+            pt = pt * st;
+            d0 = d1;
+        }
+        if (ground) {
+            f3 gp = origin + ray * end;
+            f3 gn = normalized(gp);
+            float gsc = dot(gn, sun);
+            SunGeom g = sun_geometry(k, 0.0f, gsc, bottom, ar);
+            f3 gt = solar_transmittance(k, tlut, g, 0.0f, bottom, ar);
+            f3 gl = (((arr3(k.albedo) * irradiance) * gt) * fmaxf(gsc, 0.0f)) / k.pi;
+            dl = dl + pt * gl;
+        }
+        lum = lum + dl;
+        fb = fb + df;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lum.x += __shfl_xor(lum.x, off, 64);
+        lum.y += __shfl_xor(lum.y, off, 64);
+        lum.z += __shfl_xor(lum.z, off, 64);
+        fb.x += __shfl_xor(fb.x, off, 64);
+        fb.y += __shfl_xor(fb.y, off, 64);
+        fb.z += __shfl_xor(fb.z, off, 64);
+    }
+    if (lane == 0) {
+        f3 avg_l = lum / (float)N;
+        f3 avg_f = clamp3(fb / (float)N, 0.0f, 0.95f);
+        st3(mslut, texel, maxs(avg_l / (mk(1.0f, 1.0f, 1.0f) - avg_f), 0.0f));
+    }
+}
+
 // _build_sky_view (renderer.py:1208-1302).  G lanes per texel, ordered
 // (L, T) scan over the lanes' step chunks.
 template <int G>
@@ -403,6 +490,8 @@ __global__ void __launch_bounds__(256) k_transmittance(Consts k, float bottom, f
 }
 
 // Three waves per SIMD, as for the batched build (28.6 against 30.0 us at 1024 texels).
+// (One wave per texel, as the batch runs it, takes 42 us here: 1024 waves
+// cannot fill 256 CUs, so the single state keeps four lanes per direction.)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
 k_multi_scattering(Consts k, float bottom, f3 irradiance,
                                                           float ar, const float* __restrict__ tlut,
@@ -444,19 +533,23 @@ __global__ void __launch_bounds__(256) k_transmittance_batched(const BatchState*
                           (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// Three waves per SIMD (168 VGPRs, a 56-byte spill) measure 11% faster on
-// the 256-state batch than the unconstrained two (193 VGPRs); four spill
-// 208 bytes and are slower.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
-k_multi_scattering_batched(const BatchState* __restrict__ st,
-                                                                  const float* __restrict__ tluts,
-                                                                  long t_stride,
-                                                                  float* __restrict__ msluts,
-                                                                  long m_stride) {
+// One wave per texel, four texels per block.  Four waves per SIMD (128
+// VGPRs, a 180-byte spill, 3.57 ms for 256 states) measure 2.5% faster than
+// three (168 VGPRs, a 32-byte spill, 3.66 ms); the four-lane layout of the
+// single-state build took 5.03 ms.
+#ifndef ATMO_MS_BATCH_WAVES
+#define ATMO_MS_BATCH_WAVES 4
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ATMO_MS_BATCH_WAVES, 8)))
+k_multi_scattering_batched(const BatchState* __restrict__ st, const float* __restrict__ tluts,
+                           long t_stride, float* __restrict__ msluts, long m_stride) {
     const BatchState& b = st[blockIdx.y];
-    multi_scattering_body(b.k, b.bottom, b.irradiance, b.ar_update, tluts + blockIdx.y * t_stride,
-                          msluts + blockIdx.y * m_stride, (long)blockIdx.x);
+    const long texel = (long)blockIdx.x * 4 + threadIdx.x / 64;
+    if (texel >= (long)b.k.mw * b.k.mh) return;  // wave-uniform
+    multi_scattering_wave(b.k, b.bottom, b.irradiance, b.ar_update, tluts + blockIdx.y * t_stride,
+                          msluts + blockIdx.y * m_stride, texel);
 }
+constexpr int kMsBatchTexelsPerBlock = 4;
 
 template <int G>
 // Four waves per SIMD (128 VGPRs, a 36-byte spill): 2% faster on the
@@ -820,7 +913,9 @@ void launch_batch_g(const Consts& k, const BatchState* states, unsigned count, i
                            dim3(grid_for((long)k.tw * k.th * GT), count), dim3(256), 0, s, states,
                            luts[0], strides[0]);
     if (stages & 2)
-        hipLaunchKernelGGL(k_multi_scattering_batched, dim3((unsigned)(k.mw * k.mh), count),
+        hipLaunchKernelGGL(k_multi_scattering_batched,
+                           dim3((unsigned)((k.mw * k.mh + kMsBatchTexelsPerBlock - 1) /
+                                           kMsBatchTexelsPerBlock), count),
                            dim3(256), 0, s, states, (const float*)luts[0], strides[0], luts[1],
                            strides[1]);
     if (stages & 4)

@@ -1,7 +1,10 @@
-"""Batched LUT builds: every state of a 256-state batch equals what a cold
-`update` + `prepare_frame` of its own renderer builds, a sample of states
-matches the float32 CPU restatement, graph replays are stable, and the
-reference's LUT invariants hold for all states."""
+"""Batched LUT builds: every state of a 256-state batch matches what a cold
+`update` + `prepare_frame` of its own renderer builds (the transmittance
+bitwise; the multi-scattering build marches each direction in one lane
+instead of four chunks, so it and the LUTs that read it agree to float
+reassociation), a sample of states matches the float32 CPU restatement,
+graph replays are stable, and the reference's LUT invariants hold for all
+states."""
 
 import math
 
@@ -60,15 +63,29 @@ def _single_state(pkg, s, device):
     return out
 
 
+# Batch against single-state build: the multi-scattering sums (64 directions
+# x 20 steps) differ only in association, about 1e-7 relative.
+REASSOCIATION_TOLERANCE = 2e-6
+
+
 def test_batch_equals_single_state_builds(pkg, built, device):
-    """The batched kernels run the single-state kernels' bodies: bitwise
-    equal LUTs for every 8th state (both planets)."""
+    """Every 8th state (both planets) against its own renderer: the
+    transmittance kernel body is shared, so that LUT is bitwise equal; the
+    batched multi-scattering build runs one lane per direction (the
+    single-state build four chunks combined by a scan), so MS and the
+    sky-view and aerial LUTs built from it agree to float reassociation."""
     states, b = built
     luts = [t.cpu().numpy() for t in b.luts]
+    worst = dict.fromkeys(b.NAMES[1:], 0.0)
     for i in range(0, len(states), 8):
         want = _single_state(pkg, states[i], device)
-        for name, got, ref in zip(b.NAMES, luts, want):
-            assert np.array_equal(got[i], ref), f"state {i} {name}"
+        assert np.array_equal(luts[0][i], want[0]), f"state {i} transmittance"
+        for name, got, ref in zip(b.NAMES[1:], luts[1:], want[1:]):
+            err = float(rel_error(got[i], ref).max())
+            worst[name] = max(worst[name], err)
+            assert err <= REASSOCIATION_TOLERANCE, f"state {i} {name}: {err:.2e}"
+    print("batch vs single-state max rel err: " +
+          ", ".join(f"{k} {v:.2e}" for k, v in worst.items()), flush=True)
 
 
 def test_batch_matches_cpu_oracle(built, ref32):
