@@ -164,7 +164,7 @@ int32_t atmo_set_precision(AtmoHandle* handle, int32_t precision);
 
 /* Launch-shape autotuner (no counterpart in the reference).  The
  * transmittance and sky-view builds march each texel with a group of G
- * lanes (G = 4, 8 or 16; default 8) that combine their step chunks;
+ * lanes (G = 1, 2, 4, 8 or 16; default 8) that combine their step chunks;
  * atmo_autotune times every G of both builds on `stream` with this
  * handle's LUT sizes and the given arguments (as the handle's next update
  * would pass them), keeps the fastest, and leaves T, MS and SV built with
@@ -421,10 +421,12 @@ int32_t atmo_batch_run(AtmoBatch* batch, void* stream);
  * (8).  A stage reads the LUTs its predecessors left in the batch arrays,
  * so after one atmo_batch_run any single stage can be timed on its own. */
 int32_t atmo_batch_run_stages(AtmoBatch* batch, int32_t stages, void* stream);
-/* Times the nine (G_T, G_SV) launch shapes of the batch's four stages on
- * `stream` (synchronising it), keeps the fastest for the captured graph
- * (capturing again on the next atmo_batch_run if it changed) and writes it
- * to chosen (may be NULL).  The LUT arrays hold the last trial's results. */
+/* Times the five lane-group sizes G_T of the batch's transmittance stage
+ * alone, then the five G_SV of its sky-view stage alone, on `stream`
+ * (synchronising it); keeps the fastest of each for the captured graph
+ * (capturing again on the next atmo_batch_run if they changed) and writes
+ * them to chosen (may be NULL).  The LUT arrays hold the last trial's
+ * results. */
 int32_t atmo_batch_autotune(AtmoBatch* batch, void* stream, int32_t chosen[2]);
 /* Device address and shape of the batch's LUT array `which`
  * (ATMO_BUF_TRANSMITTANCE .. ATMO_BUF_AERIAL_SCATTERING): shape[0] = count,

@@ -111,9 +111,10 @@ class LutBatch:
                                   .decode(errors="replace"))
 
     def autotune(self, stream=None) -> tuple[int, int]:
-        """Time the nine (lanes per T texel, lanes per sky-view texel)
-        launch shapes of the batch's stages and keep the fastest for the
-        captured graph (atmo_batch_autotune).  Synchronises the stream."""
+        """Time the five lanes-per-texel sizes (1, 2, 4, 8, 16) of the
+        batch's transmittance stage, then of its sky-view stage, and keep the
+        fastest of each for the captured graph (atmo_batch_autotune).
+        Synchronises the stream."""
         import torch
 
         s = stream if stream is not None else torch.cuda.current_stream(self.device)

@@ -32,8 +32,8 @@ struct AtmoHandle {
     int queue_parity;
     bool queue_reset;  // a composite launch failed: clear both counters first
     int precision;     // ATMO_PRECISION_REFERENCE or ATMO_PRECISION_ROBUST
-    int group_t;       // lanes per transmittance texel (4, 8, 16), autotuned
-    int group_sv;      // lanes per sky-view texel (4, 8, 16), autotuned
+    int group_t;       // lanes per transmittance texel (1, 2, 4, 8, 16), autotuned
+    int group_sv;      // lanes per sky-view texel (1, 2, 4, 8, 16), autotuned
     AtmoLutCache cache;  // the host policy of atmo_update / atmo_prepare_frame
     char error[512];
 };

@@ -663,7 +663,7 @@ int32_t atmo_composite_noalloc(const AtmoParams* params, const float* transmitta
 
 namespace {
 
-constexpr int kGroups[3] = {4, 8, 16};
+constexpr int kGroups[5] = {1, 2, 4, 8, 16};
 
 // Mean duration in ms of `launch` on `stream` over `reps` runs after one
 // warm-up run; a negative value if a HIP call failed.
@@ -743,7 +743,7 @@ int32_t atmo_get_tuning(AtmoHandle* h, int32_t tuning[2]) {
 int32_t atmo_set_tuning(AtmoHandle* h, const int32_t tuning[2]) {
     if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
     if (!tuning || !atmo::ref::valid_group(tuning[0]) || !atmo::ref::valid_group(tuning[1])) {
-        set_error(h, "atmo_set_tuning: lanes per texel must be 4, 8 or 16");
+        set_error(h, "atmo_set_tuning: lanes per texel must be 1, 2, 4, 8 or 16");
         return ATMO_EINVAL;
     }
     h->group_t = tuning[0];
@@ -964,20 +964,33 @@ int32_t atmo_batch_run_stages(AtmoBatch* b, int32_t stages, void* stream) {
 int32_t atmo_batch_autotune(AtmoBatch* b, void* stream, int32_t chosen[2]) {
     if (!batch_ok(b)) return ATMO_EHANDLE;
     hipStream_t s = (hipStream_t)stream;
-    float best = 1.0e30f;
+    // The two shapes are independent: time the transmittance stage alone
+    // for each G_T, then (with T and MS built) the sky-view stage alone for
+    // each G_SV.
     int bt = b->group_t, bs = b->group_sv;
-    for (int gt : kGroups)
-        for (int gs : kGroups) {
-            float ms = time_launch(s, 2, [&] {
-                atmo::ref::launch_batch(b->k0, b->states, (unsigned)b->count, b->width, b->height,
-                                        b->luts, b->strides, s, gt, gs);
-            });
-            if (ms < 0.0f) {
-                batch_error(b, "atmo_batch_autotune: launch failed");
-                return ATMO_EHIP;
-            }
-            if (ms < best) best = ms, bt = gt, bs = gs;
+    const auto run = [&](int gt, int gs, int stages) {
+        atmo::ref::launch_batch(b->k0, b->states, (unsigned)b->count, b->width, b->height, b->luts,
+                                b->strides, s, gt, gs, stages);
+    };
+    float best = 1.0e30f;
+    for (int gt : kGroups) {
+        float ms = time_launch(s, 3, [&] { run(gt, bs, 1); });
+        if (ms < 0.0f) {
+            batch_error(b, "atmo_batch_autotune: launch failed");
+            return ATMO_EHIP;
         }
+        if (ms < best) best = ms, bt = gt;
+    }
+    run(bt, bs, 3);
+    best = 1.0e30f;
+    for (int gs : kGroups) {
+        float ms = time_launch(s, 3, [&] { run(bt, gs, 4); });
+        if (ms < 0.0f) {
+            batch_error(b, "atmo_batch_autotune: launch failed");
+            return ATMO_EHIP;
+        }
+        if (ms < best) best = ms, bs = gs;
+    }
     if (bt != b->group_t || bs != b->group_sv) {
         // the captured graph holds the old shapes: capture again on the next run
         if (b->exec) (void)hipGraphExecDestroy(b->exec);

@@ -827,10 +827,11 @@ __global__ void __launch_bounds__(256) k_composite_diagnostic(Consts k, Composit
 
 inline unsigned grid_for(long threads) { return (unsigned)((threads + 255) / 256); }
 
-// Lanes per texel of the transmittance and sky-view builds: a multiple of
-// 2 that divides the wave (4, 8 or 16), chosen per handle or batch by the
-// autotuner (default 8).
-inline bool valid_group(int g) { return g == 4 || g == 8 || g == 16; }
+// Lanes per texel of the transmittance and sky-view builds: a power of 2
+// up to 16 (1, 2, 4, 8 or 16), chosen per handle or batch by the autotuner
+// (default 8).  A single state's few thousand texels need several lanes each
+// to fill the GPU; a batch of states fills it with one.
+inline bool valid_group(int g) { return g == 1 || g == 2 || g == 4 || g == 8 || g == 16; }
 
 template <int G>
 void launch_transmittance_g(const Consts& k, float bottom, float* tlut, hipStream_t s) {
@@ -839,12 +840,13 @@ void launch_transmittance_g(const Consts& k, float bottom, float* tlut, hipStrea
 }
 
 void launch_transmittance(const Consts& k, float bottom, float* tlut, hipStream_t s, int g) {
-    if (g == 4)
-        launch_transmittance_g<4>(k, bottom, tlut, s);
-    else if (g == 16)
-        launch_transmittance_g<16>(k, bottom, tlut, s);
-    else
-        launch_transmittance_g<8>(k, bottom, tlut, s);
+    switch (g) {
+        case 1: launch_transmittance_g<1>(k, bottom, tlut, s); break;
+        case 2: launch_transmittance_g<2>(k, bottom, tlut, s); break;
+        case 4: launch_transmittance_g<4>(k, bottom, tlut, s); break;
+        case 16: launch_transmittance_g<16>(k, bottom, tlut, s); break;
+        default: launch_transmittance_g<8>(k, bottom, tlut, s); break;
+    }
 }
 
 void launch_multi_scattering(const Consts& k, float bottom, f3 irradiance, float ar,
@@ -864,12 +866,13 @@ void launch_sky_view_g(const Consts& k, float bottom, float cam_alt, float sun_z
 void launch_sky_view(const Consts& k, float bottom, float cam_alt, float sun_zc, f3 irradiance,
                      float ar, const float* tlut, const float* mslut, float* svlut, hipStream_t s,
                      int g) {
-    if (g == 4)
-        launch_sky_view_g<4>(k, bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut, svlut, s);
-    else if (g == 16)
-        launch_sky_view_g<16>(k, bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut, svlut, s);
-    else
-        launch_sky_view_g<8>(k, bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut, svlut, s);
+    switch (g) {
+        case 1: launch_sky_view_g<1>(k, bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut, svlut, s); break;
+        case 2: launch_sky_view_g<2>(k, bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut, svlut, s); break;
+        case 4: launch_sky_view_g<4>(k, bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut, svlut, s); break;
+        case 16: launch_sky_view_g<16>(k, bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut, svlut, s); break;
+        default: launch_sky_view_g<8>(k, bottom, cam_alt, sun_zc, irradiance, ar, tlut, mslut, svlut, s); break;
+    }
 }
 
 void launch_aerial(const Consts& k, int width, int height, float bottom, float cam_alt, f3 sun,
@@ -932,23 +935,25 @@ template <int GT>
 void launch_batch_gt(const Consts& k, const BatchState* states, unsigned count, int width,
                      int height, float* const luts[4], const long strides[4], hipStream_t s,
                      int g_sv, int stages) {
-    if (g_sv == 4)
-        launch_batch_g<GT, 4>(k, states, count, width, height, luts, strides, s, stages);
-    else if (g_sv == 16)
-        launch_batch_g<GT, 16>(k, states, count, width, height, luts, strides, s, stages);
-    else
-        launch_batch_g<GT, 8>(k, states, count, width, height, luts, strides, s, stages);
+    switch (g_sv) {
+        case 1: launch_batch_g<GT, 1>(k, states, count, width, height, luts, strides, s, stages); break;
+        case 2: launch_batch_g<GT, 2>(k, states, count, width, height, luts, strides, s, stages); break;
+        case 4: launch_batch_g<GT, 4>(k, states, count, width, height, luts, strides, s, stages); break;
+        case 16: launch_batch_g<GT, 16>(k, states, count, width, height, luts, strides, s, stages); break;
+        default: launch_batch_g<GT, 8>(k, states, count, width, height, luts, strides, s, stages); break;
+    }
 }
 
 void launch_batch(const Consts& k, const BatchState* states, unsigned count, int width, int height,
                   float* const luts[4], const long strides[4], hipStream_t s, int g_t, int g_sv,
                   int stages) {
-    if (g_t == 4)
-        launch_batch_gt<4>(k, states, count, width, height, luts, strides, s, g_sv, stages);
-    else if (g_t == 16)
-        launch_batch_gt<16>(k, states, count, width, height, luts, strides, s, g_sv, stages);
-    else
-        launch_batch_gt<8>(k, states, count, width, height, luts, strides, s, g_sv, stages);
+    switch (g_t) {
+        case 1: launch_batch_gt<1>(k, states, count, width, height, luts, strides, s, g_sv, stages); break;
+        case 2: launch_batch_gt<2>(k, states, count, width, height, luts, strides, s, g_sv, stages); break;
+        case 4: launch_batch_gt<4>(k, states, count, width, height, luts, strides, s, g_sv, stages); break;
+        case 16: launch_batch_gt<16>(k, states, count, width, height, luts, strides, s, g_sv, stages); break;
+        default: launch_batch_gt<8>(k, states, count, width, height, luts, strides, s, g_sv, stages); break;
+    }
 }
 #endif
 

@@ -262,7 +262,7 @@ class AtmosphereRenderer:
     def autotune(self, planet_radius_m: float, camera_radius_m: float, sun_local,
                  solar_irradiance, sun_angular_radius_degrees: float) -> tuple[int, int]:
         """Time the transmittance and sky-view launch shapes (lanes per
-        texel 4, 8, 16) with this renderer's LUT sizes and keep the fastest
+        texel 1, 2, 4, 8, 16) with this renderer's LUT sizes and keep the fastest
         (atmo_autotune).  Arguments as for `update`; the LUTs are rebuilt,
         so every cache key is reset.  Returns (lanes per T texel, lanes per
         sky-view texel)."""

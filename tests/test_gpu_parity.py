@@ -261,7 +261,8 @@ def test_autotuned_shapes_keep_parity(pkg, scene, ref32, device):
     r = pkg.AtmosphereRenderer(64, 36, cfg, device)
     chosen = r.autotune(R, r_cam, sun, E, scene.SUN_ANGULAR_RADIUS_DEG)
     print(f"autotune: lanes per T texel {chosen[0]}, per SV texel {chosen[1]}", flush=True)
-    assert chosen in [(gt, gs) for gt in (4, 8, 16) for gs in (4, 8, 16)]
+    groups = (1, 2, 4, 8, 16)
+    assert chosen in [(gt, gs) for gt in groups for gs in groups]
     assert r.tuning == chosen
     ar = float(np.float32(math.radians(scene.SUN_ANGULAR_RADIUS_DEG)))
     t = ref32.build_transmittance(cfg, R)
@@ -269,7 +270,7 @@ def test_autotuned_shapes_keep_parity(pkg, scene, ref32, device):
     sun_n = np.asarray(sun) / np.linalg.norm(sun)
     sv = ref32.build_sky_view(cfg, R, float(np.float32(r_cam - R)), float(np.float32(sun_n[1])),
                               E, ar, t, ms)
-    for g in (4, 8, 16):
+    for g in groups:
         r.tuning = (g, g)
         r.update(R, r_cam, sun, E, scene.SUN_ANGULAR_RADIUS_DEG)
         import torch
@@ -280,6 +281,6 @@ def test_autotuned_shapes_keep_parity(pkg, scene, ref32, device):
                                 ("SV", r.sky_view_lut.to_numpy(), sv)):
             e = summary(rel_error(got, want))
             assert e["max"] <= PLAIN_TOLERANCE, (g, name, e["max"])
-    with pytest.raises(ValueError, match="4, 8 or 16"):
+    with pytest.raises(ValueError, match="1, 2, 4, 8 or 16"):
         r.tuning = (3, 8)
     r.close()
