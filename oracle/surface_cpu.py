@@ -61,19 +61,22 @@ def make_params(params: dict) -> ShadeParams:
     return p
 
 
-def _lib() -> ctypes.CDLL:
-    if not LIBRARY.exists():
-        build()
-    lib = ctypes.CDLL(str(LIBRARY))
+def _lib(library: Path | None = None) -> ctypes.CDLL:
+    if library is None:
+        if not LIBRARY.exists():
+            build()
+        library = LIBRARY
+    lib = ctypes.CDLL(str(library))
     lib.surface_shade.argtypes = [ctypes.POINTER(ShadeFields), ctypes.POINTER(ShadeParams),
                                   ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]
     lib.surface_shade.restype = None
     return lib
 
 
-def shade(fields: dict, params: dict, width: int, height: int):
-    """Returns (surface_hdr (W, H, 3) float32, path (W, H) int32)."""
-    lib = _lib()
+def shade(fields: dict, params: dict, width: int, height: int, library: Path | None = None):
+    """Returns (surface_hdr (W, H, 3) float32, path (W, H) int32).  `library`
+    selects another build of surface_cpu.c (the tests' planted mistakes)."""
+    lib = _lib(library)
     arrays = {k: np.ascontiguousarray(v) for k, v in fields.items()}
     out = np.empty((width, height, 3), np.float32)
     path = np.empty((width, height), np.int32)

@@ -16,9 +16,8 @@ restatement of the reference's `_shade_surface`
     sample: every pixel's seabed albedo, recovered from the layer formula,
     is its own terrain albedo (this pins the projection and the field
     indexing of the lookup).
-Snell bending at oblique incidence, the Smith G term away from normal
-incidence and the offset clamp are checked only through GPU parity with
-this oracle.
+Oblique incidence (GGX, Smith G and Fresnel away from the normal, Snell
+bending, the offset clamp) is pinned in test_surface_pins.py.
 """
 
 import math
