@@ -17,7 +17,14 @@
 // (1 - exp(-x)) / s for small x together with the density exp that feeds
 // x, the sky-view horizon coordinate.  Where nothing downstream amplifies a
 // last-bit difference (exp of long segments) the hardware-rate function is
-// used; each such site says so.
+// used; each such site says so.  Each correctly rounded site was switched to
+// the hardware-rate function in turn and the GPU parity tests run: four
+// fail without it (the short-segment exp, the penumbra zeniths, the
+// representative cosine above the band, the sky-view sampler's acos), the
+// density exp brings the twilight pixel to the 1e-4 bar, and the disk
+// visibility quadruples its worst error.  The Mie phase's power and the
+// sun-disk distance left every test unchanged but saved no measurable time,
+// so they stay correctly rounded like the CPU restatement.
 //
 // Frame- and LUT-sized arrays use the reference's Taichi field layout:
 // (X, Y[, Z], 3) float32, element [x, y] at (x * Y + y) * 3.
