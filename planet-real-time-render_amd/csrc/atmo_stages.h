@@ -489,10 +489,14 @@ __global__ void __launch_bounds__(256) k_transmittance(Consts k, float bottom, f
     transmittance_body<G>(k, bottom, tlut, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// Three waves per SIMD, as for the batched build (28.6 against 30.0 us at 1024 texels).
-// (One wave per texel, as the batch runs it, takes 42 us here: 1024 waves
-// cannot fill 256 CUs, so the single state keeps four lanes per direction.)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
+// Three waves per SIMD at 1024 texels: 28.1 us, against 29.2 us at two and
+// 38.5 us at four (128 VGPRs, a 196-byte spill inside the step loop).  One
+// wave per texel, as the batch runs it, takes 42 us here: 1024 waves cannot
+// fill 256 CUs, so the single state keeps four lanes per direction.
+#ifndef ATMO_MS_WAVES
+#define ATMO_MS_WAVES 3
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ATMO_MS_WAVES, 8)))
 k_multi_scattering(Consts k, float bottom, f3 irradiance,
                                                           float ar, const float* __restrict__ tlut,
                                                           float* __restrict__ mslut) {
