@@ -137,6 +137,33 @@ def test_terminator_frames_parity(pkg, scene, ref32, device):
             check(f"terminator {k} {name}", got[name], want[name])
 
 
+@pytest.mark.parametrize("width, height", [(1, 1), (1, 7), (5, 3), (17, 13), (255, 1), (1, 257),
+                                           (16, 17)])
+def test_tiny_and_ragged_frames_parity(pkg, scene, ref32, device, width, height):
+    """Frames of one partial 256-pixel block, or a few pixels past a block
+    boundary: the composite's compaction, LDS output tile and last-wave
+    store must write exactly the frame's pixels.  COMPOSITE and the camera
+    transmittance view, reference and robust precision, and a terminator
+    case whose deferred pixels go through the queue."""
+    inputs = scene.make_frame(width=width, height=height, config_index=1, seed_offset=width + height)
+    got = gpu_frame(pkg, inputs, device)
+    want = ref32.frame(inputs)
+    for name in ("surface_sun_transmittance", "surface_sky_radiance", "hdr"):
+        check(f"{width}x{height} {name}", got[name], want[name])
+    check(f"{width}x{height} view 2", gpu_frame(pkg, inputs, device, view=2)["hdr"],
+          ref32.frame(inputs, view=2)["hdr"])
+    from oracle.ref_cpu import RefCpu
+
+    check(f"{width}x{height} robust hdr", gpu_frame(pkg, inputs, device, precision="robust")["hdr"],
+          RefCpu("f32r").frame(inputs)["hdr"])
+    inputs.sun_local = scene.sun_from_angles(90.5, 0.3)
+    inputs.lighting = pkg.LightingState(inputs.sun_local, 0.266, scene.SOLAR_IRRADIANCE)
+    inputs.camera_radius_m = inputs.planet_radius_m + 20_000.0
+    got, want = gpu_frame(pkg, inputs, device), ref32.frame(inputs)
+    print(f"{width}x{height} terminator: refined intervals {want['refined_intervals']}", flush=True)
+    check(f"{width}x{height} terminator hdr", got["hdr"], want["hdr"])
+
+
 def test_small_planet_invariants_and_lut_cache(pkg, scene, device):
     """The reference's CPU render test on GPU: tiny LUTs on a 50 km planet
     (tests/test_m2_terrain.py:252-271), then the freeze / resume state
