@@ -112,6 +112,10 @@ enum {
 
 typedef struct AtmoHandle AtmoHandle;
 
+/* The defaults of `AtmosphereConfig` (config.py:33-59), for hosts that
+ * build an AtmoParams without the Python facade. */
+int32_t atmo_default_params(AtmoParams* out);
+
 /* Range checks of `AtmosphereConfig.__post_init__` (config.py:61-128).
  * Writes the reference's ValueError message into `message` on failure. */
 int32_t atmo_validate_params(const AtmoParams* params, char* message,

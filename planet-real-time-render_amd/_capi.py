@@ -34,6 +34,7 @@ NOALLOC_MAX_PIXELS = 8388608  # ATMO_NOALLOC_MAX_PIXELS
 
 # Every symbol include/atmo.h declares, checked by the CPU test suite.
 EXPORTED_SYMBOLS = (
+    "atmo_default_params",
     "atmo_validate_params",
     "atmo_create",
     "atmo_destroy",
@@ -274,6 +275,7 @@ _PF3 = ctypes.POINTER(ctypes.c_float)
 
 
 def _signatures(lib: ctypes.CDLL) -> None:
+    lib.atmo_default_params.argtypes = [ctypes.POINTER(AtmoParams)]
     lib.atmo_validate_params.argtypes = [ctypes.POINTER(AtmoParams), ctypes.c_char_p, _I]
     lib.atmo_create.argtypes = [ctypes.POINTER(AtmoParams), _I, _I, ctypes.POINTER(_P)]
     lib.atmo_destroy.argtypes = [_P]

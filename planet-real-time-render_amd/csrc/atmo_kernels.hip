@@ -221,6 +221,45 @@ void free_buffers(AtmoHandle* h) {
 
 extern "C" {
 
+int32_t atmo_default_params(AtmoParams* out) {
+    if (!out) return ATMO_EINVAL;
+    // atmosphere/config.py:33-59
+    AtmoParams p = {};
+    p.top_altitude_m = 100000.0;
+    const double rayleigh[3] = {5.802e-6, 13.558e-6, 33.100e-6};
+    const double absorption[3] = {0.650e-6, 1.881e-6, 0.085e-6};
+    for (int c = 0; c < 3; ++c) {
+        p.rayleigh_scattering_per_m[c] = rayleigh[c];
+        p.mie_scattering_per_m[c] = 3.996e-6;
+        p.mie_extinction_per_m[c] = 4.440e-6;
+        p.absorption_extinction_per_m[c] = absorption[c];
+        p.ground_albedo[c] = 0.10;
+    }
+    p.rayleigh_scale_height_m = 8000.0;
+    p.mie_scale_height_m = 1200.0;
+    p.mie_phase_g = 0.80;
+    p.absorption_peak_altitude_m = 25000.0;
+    p.absorption_half_width_m = 15000.0;
+    p.transmittance_lut_width = 256;
+    p.transmittance_lut_height = 64;
+    p.multi_scattering_lut_width = 32;
+    p.multi_scattering_lut_height = 32;
+    p.sky_view_lut_width = 192;
+    p.sky_view_lut_height = 256;
+    p.aerial_lut_width = 32;
+    p.aerial_lut_height = 18;
+    p.aerial_lut_depth = 32;
+    p.transmittance_steps = 40;
+    p.multi_scattering_directions = 16;
+    p.multi_scattering_steps = 12;
+    p.sky_view_steps = 24;
+    p.aerial_steps_per_slice = 2;
+    p.aerial_horizon_raymarch_steps = 12;
+    p.aerial_terminator_substeps = 8;
+    *out = p;
+    return ATMO_OK;
+}
+
 int32_t atmo_validate_params(const AtmoParams* params, char* message, int32_t capacity) {
     if (!params) {
         if (message && capacity > 0) snprintf(message, (size_t)capacity, "params is NULL");
