@@ -582,7 +582,22 @@ __global__ void __launch_bounds__(256) k_aerial_batched(const BatchState* __rest
 }
 #endif  // !ATMO_ROBUST
 
-// _prepare_surface_lighting (renderer.py:1422-1472).  One thread per pixel.
+// A pixel's three floats in a block's LDS tile of 256 pixels.
+__device__ __forceinline__ void tile_put(float* tile, uint32_t l, f3 v) {
+    tile[3 * l + 0] = v.x;
+    tile[3 * l + 1] = v.y;
+    tile[3 * l + 2] = v.z;
+}
+__device__ __forceinline__ f3 tile_get(const float* tile, uint32_t l) {
+    return f3{tile[3 * l], tile[3 * l + 1], tile[3 * l + 2]};
+}
+
+// _prepare_surface_lighting (renderer.py:1422-1472).  One block per 256
+// pixels.  The surface pixels are compacted onto the leading threads, so
+// every lane of a lighting wave lights a surface pixel (with a per-pixel sky
+// mask, a sky lane would otherwise idle through the whole branch); sky and
+// out-of-frame lanes only hold the constants.  The outputs go through two
+// LDS tiles in pixel order, so each thread then stores its own pixel.
 __global__ void __launch_bounds__(256) k_prepare(Consts k, long npix,
                                                  const float* __restrict__ position_view,
                                                  const int32_t* __restrict__ surface_id,
@@ -592,13 +607,21 @@ __global__ void __launch_bounds__(256) k_prepare(Consts k, long npix,
                                                  const float* __restrict__ mslut,
                                                  float* __restrict__ sun_t,
                                                  float* __restrict__ sky_l) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;  // npix <= 2^28
-    if (i >= npix) return;
-    f3 solar = mk(1.0f, 1.0f, 1.0f);
-    f3 sky = mk(0.0f, 0.0f, 0.0f);
-    if (surface_id[i] >= 0) {
+    __shared__ int list[256];
+    __shared__ int wave_counts[4];
+    __shared__ float tile_t[256 * 3], tile_l[256 * 3];
+    const uint32_t base = blockIdx.x * 256u;
+    const uint32_t i = base + threadIdx.x;  // npix <= 2^28
+    const bool live = i < npix;
+    tile_put(tile_t, threadIdx.x, mk(1.0f, 1.0f, 1.0f));
+    tile_put(tile_l, threadIdx.x, mk(0.0f, 0.0f, 0.0f));
+    // (the compaction's barriers order these stores before the surface ones)
+    const int count = compact_block(live && surface_id[i] >= 0, list, wave_counts);
+    if ((int)threadIdx.x < count) {
+        const uint32_t l = list[threadIdx.x];
+        const uint32_t p = base + l;
         f3 camera = mk(0.0f, bottom + cam_alt, 0.0f);
-        f3 vp = pix3(position_view, i);
+        f3 vp = pix3(position_view, p);
         f3 local = (right * vp.x + up * vp.y) + forward * vp.z;
         f3 planet = camera + local;
         f3 radial = div3(planet, fmaxf(norm(planet), 1.0f));
@@ -622,11 +645,14 @@ __global__ void __launch_bounds__(256) k_prepare(Consts k, long npix,
 #endif
         float h = altitude_on_ray(cam_alt, local_ray.y, sd, bottom);
         SunGeom g = sun_geometry(k, h, sc, bottom, ar);
-        solar = solar_transmittance(k, tlut, g, h, bottom, ar);
-        sky = sample_multi_scattering(k, mslut, h, sc, bottom);
+        tile_put(tile_t, l, solar_transmittance(k, tlut, g, h, bottom, ar));
+        tile_put(tile_l, l, maxs(sample_multi_scattering(k, mslut, h, sc, bottom), 0.0f));
     }
-    put3(sun_t, i, solar);
-    put3(sky_l, i, maxs(sky, 0.0f));
+    __syncthreads();
+    if (live) {
+        put3(sun_t, i, tile_get(tile_t, threadIdx.x));
+        put3(sky_l, i, tile_get(tile_l, threadIdx.x));
+    }
 }
 
 
@@ -656,11 +682,6 @@ __device__ __forceinline__ f3 composite_sky_pixel(const Consts& k, const Composi
 // finish tens of microseconds apart: written directly, a line evicted
 // between the two writes was written back twice.  Waves that finish early
 // still exit at once: there is no block-wide barrier.
-__device__ __forceinline__ void tile_put(float* tile, uint32_t l, f3 v) {
-    tile[3 * l + 0] = v.x;
-    tile[3 * l + 1] = v.y;
-    tile[3 * l + 2] = v.z;
-}
 __device__ __forceinline__ void store_tile_by_last_wave(const float* tile, int* waves_done,
                                                         float* __restrict__ hdr, uint32_t base,
                                                         uint32_t npix) {
