@@ -48,7 +48,11 @@
 namespace {
 
 constexpr uint32_t kPostMagic = 0x504f5354u;  // "POST"
-constexpr int kTile = 48;                     // half-resolution texels per tile side
+// Half-resolution texels per tile side.  At 1920x1080 the bloom frame
+// (4 and 8 pass pairs) took 38.6 and 57.9 us with 48 x 48 tiles of 1024
+// threads, 38.5 and 67.4 with 32 x 32, 43.7 and 66.5 with 40 x 40, and
+// longer with 512 threads: fewer, larger tiles re-read less halo.
+constexpr int kTile = 48;
 constexpr int kThreads = 256;
 constexpr int kBlurThreads = 1024;
 constexpr int kXcds = 8;
