@@ -32,6 +32,7 @@ from pathlib import Path
 import numpy as np
 
 REPO = Path(__file__).resolve().parent
+
 sys.path.insert(0, str(REPO))
 
 import __graft_entry__ as entry  # noqa: E402
@@ -42,7 +43,7 @@ FP32_NONPACKED_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic work per unit (FP32 ops F, transcendental-class ops T), the
-# hand count of the reference's source per stage; see DESIGN.md.
+# hand count of the reference's source per stage.
 COST = {
     "surface_pixel": (6746.0, 693.0),
     "sky_pixel": (328.0, 32.0),
@@ -71,6 +72,12 @@ COMPOSITE_BYTES_PER_PIXEL = 4 + 12 + 12 + 12
 COMPOSITE_TRAFFIC_BYTES_1080P = 108.82e6
 # Memory moves whole cache lines (128 B on MI355X's L2).
 CACHE_LINE_BYTES = 128
+
+
+def display_path(path) -> str:
+    """The path relative to the repository root when it lies inside it."""
+    path = Path(path).resolve()
+    return str(path.relative_to(REPO)) if path.is_relative_to(REPO) else str(path)
 
 
 def composite_line_bytes(surface_id) -> int:
@@ -579,7 +586,7 @@ def main() -> int:
         log(f"surrogate: {s_wall:.4f} ms/frame (wall), composite "
             f"{s_stage_ms['composite_surrogate'] * 1e3:.2f} us")
         surrogate = {"wall_ms": s_wall, "frame_ms": s_frame_ms, "stage_ms": s_stage_ms,
-                     "hdr": sur_hdr, "weights": str(weights)}
+                     "hdr": sur_hdr, "weights": display_path(weights)}
         net.close()
     else:
         log(f"no surrogate weights at {weights}.bin: surrogate frame not measured")
