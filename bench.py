@@ -585,8 +585,14 @@ def main() -> int:
         sur_hdr = frame.hdr.cpu().numpy()
         log(f"surrogate: {s_wall:.4f} ms/frame (wall), composite "
             f"{s_stage_ms['composite_surrogate'] * 1e3:.2f} us")
+        # surface pixels the surrogate composite left to the plain integrator
+        # (out of the network's domain or routed by route_share)
+        plain_share = frame.r.composite_queued_pixels() / max(int((inputs.surface_id >= 0).sum()), 1)
+        log(f"surrogate: route_share {net.route_share}, {plain_share:.3f} of surface pixels "
+            f"on the plain path")
         surrogate = {"wall_ms": s_wall, "frame_ms": s_frame_ms, "stage_ms": s_stage_ms,
-                     "hdr": sur_hdr, "weights": display_path(weights)}
+                     "hdr": sur_hdr, "weights": display_path(weights),
+                     "route_share": net.route_share, "plain_path_share": plain_share}
         net.close()
     else:
         log(f"no surrogate weights at {weights}.bin: surrogate frame not measured")
@@ -792,6 +798,8 @@ def main() -> int:
             "surrogate_stage_us": {k: round(v * 1e3, 2) for k, v in surrogate["stage_ms"].items()},
             "surrogate_speedup": round(ms_per_step / surrogate["wall_ms"], 3),
             "surrogate_p99_gate_met": gate,
+            "surrogate_route_share": surrogate["route_share"],
+            "surrogate_surface_pixels_on_plain_path": round(surrogate["plain_path_share"], 4),
             "surrogate_weights": surrogate["weights"],
         })
     result = {

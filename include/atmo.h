@@ -236,6 +236,13 @@ int32_t atmo_composite(
     float tangent_half_fov, float star_contrast_start,
     float star_contrast_end, int32_t diagnostic_view, void* stream);
 
+/* Number of surface pixels the handle's last COMPOSITE-view call queued for
+ * the plain integrator's second pass: the pixels that need the terminator
+ * refinement in atmo_composite, the out-of-domain and routed pixels in
+ * atmo_composite_surrogate.  Waits for `stream`.  No counterpart in the
+ * reference (a diagnostic of this library's deferred-pixel queue). */
+int32_t atmo_composite_queued(AtmoHandle* handle, int64_t* count, void* stream);
+
 /* ------------------------------------------------------------------------
  * The reference's host methods in C.  `AtmosphereRenderer.update`,
  * `prepare_frame`, `composite` and `set_luts_frozen` (renderer.py:2057-2238)
@@ -466,7 +473,11 @@ int32_t atmo_batch_get_buffer(AtmoBatch* batch, int32_t which, void** device_ptr
  * L_c = exp(v_c), zero below 1.5 eps_inscatter; T_c as in mode 0.
  * exact_transmittance = 1 makes the surrogate composite evaluate T with the
  * plain two-sample formula (renderer.py:1604-1665) instead of the network's
- * T outputs. */
+ * T outputs.
+ * route_share in (0, 1) sends a surface pixel to the plain integrator when,
+ * in any channel, the learned in-scatter E L exceeds that share of the
+ * pixel E L + scene T: where in-scatter carries the pixel, the network's
+ * relative error would show in it.  0 routes no pixel on this criterion. */
 typedef struct AtmoSurrogateTransform {
     float altitude_max_m;
     float altitude_scale_m;
@@ -480,6 +491,7 @@ typedef struct AtmoSurrogateTransform {
     float planet_radius_m;
     float cosine_stretch;
     int32_t exact_transmittance;
+    float route_share;
 } AtmoSurrogateTransform;
 
 typedef struct AtmoMlp AtmoMlp;
@@ -527,7 +539,9 @@ int32_t atmo_surrogate_labels(AtmoHandle* handle, const float* raw, int64_t n,
 
 /* The COMPOSITE view of atmo_composite with the surrogate in place of the
  * surface branch's in-scatter and transmittance for in-domain pixels; same
- * arguments as atmo_composite without diagnostic_view. */
+ * arguments as atmo_composite without diagnostic_view.  Out-of-domain
+ * pixels, and pixels routed by the transform's route_share, are computed by
+ * the plain integrator, bitwise as atmo_composite computes them. */
 int32_t atmo_composite_surrogate(
     AtmoHandle* handle, AtmoMlp* mlp, const float* scene_hdr,
     const float* space_hdr, const int32_t* surface_id, const float* position_view,

@@ -51,6 +51,7 @@ EXPORTED_SYMBOLS = (
     "atmo_build_aerial_perspective",
     "atmo_prepare_surface_lighting",
     "atmo_composite",
+    "atmo_composite_queued",
     "atmo_composite_noalloc",
     "atmo_lut_cache_init",
     "atmo_lut_cache_set_frozen",
@@ -115,6 +116,7 @@ class AtmoSurrogateTransform(ctypes.Structure):
         ("planet_radius_m", ctypes.c_float),
         ("cosine_stretch", ctypes.c_float),
         ("exact_transmittance", ctypes.c_int32),
+        ("route_share", ctypes.c_float),
     ]
 
 
@@ -301,6 +303,7 @@ def _signatures(lib: ctypes.CDLL) -> None:
         _P, _P, _P, _P, _P, _P, _F, _F, _F, _PF3, _PF3, _PF3, _F, _PF3, _PF3, _PF3,
         _F, _F, _F, _I, _P,
     ]
+    lib.atmo_composite_queued.argtypes = [_P, ctypes.POINTER(ctypes.c_int64), _P]
     lib.atmo_composite_noalloc.argtypes = [
         ctypes.POINTER(AtmoParams), _P, _P, _P, _P, _I, _I,
         _P, _P, _P, _P, _P, _F, _F, _F, _PF3, _PF3, _PF3, _F, _PF3, _PF3, _PF3,

@@ -92,6 +92,12 @@ struct BatchState {
 // deferred to a.queue, in the handle's precision variant (atmo_kernels.hip).
 int32_t atmo_launch_composite_list(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream);
 
+// Launch the coarse-only surface branch for every pixel of a.queue (the
+// surrogate composite's fallback): finished pixels are written and their
+// entries set to -1, the rest are left for atmo_launch_composite_list
+// (atmo_kernels.hip).
+int32_t atmo_launch_composite_queue(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream);
+
 // Clears both counters of a deferred-pixel queue with a one-wave kernel
 // (no memset node, so the launch can sit in a captured graph).
 void atmo_launch_clear_queue(int* queue, hipStream_t stream);

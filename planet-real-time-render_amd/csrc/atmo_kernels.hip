@@ -489,6 +489,11 @@ int32_t atmo_launch_composite_list(AtmoHandle* h, const CompositeArgs& a, hipStr
     return atmo_launch_status(h, "k_composite_list");
 }
 
+int32_t atmo_launch_composite_queue(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream) {
+    ATMO_STAGE(h, launch_composite_queue, h->k, a, stream);
+    return atmo_launch_status(h, "k_composite_queue_coarse");
+}
+
 namespace {
 
 __global__ void k_clear_queue(int* queue) {
@@ -574,6 +579,28 @@ int32_t atmo_composite(AtmoHandle* h, const float* scene_hdr, const float* space
         h->queue_reset = true;
         return ATMO_EHIP;
     }
+    return ATMO_OK;
+}
+
+int32_t atmo_composite_queued(AtmoHandle* h, int64_t* count, void* stream) {
+    if (!atmo_handle_ok(h)) return ATMO_EHANDLE;
+    if (!count) {
+        set_error(h, "atmo_composite_queued: NULL count");
+        return ATMO_EINVAL;
+    }
+    // atmo_begin_queue hands a call the current parity and then flips it,
+    // so the last call's counter is the other one; that call's list pass
+    // cleared only the counter of the call after it
+    int n = 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (hipMemcpyAsync(&n, h->queue + (h->queue_parity ^ 1), sizeof(int), hipMemcpyDeviceToHost,
+                       s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        set_error(h, "atmo_composite_queued: reading the queue counter failed");
+        return ATMO_EHIP;
+    }
+    const long capacity = (long)h->width * h->height;
+    *count = n < 0 ? 0 : (n > capacity ? capacity : n);
     return ATMO_OK;
 }
 

@@ -24,7 +24,8 @@
                         float ar, f3 right, f3 up, f3 forward, const float* tlut,                \
                         const float* mslut, float* sun_t, float* sky_l, hipStream_t s);          \
     void launch_composite(const Consts& k, const CompositeArgs& a, hipStream_t s);              \
-    void launch_composite_list(const Consts& k, const CompositeArgs& a, hipStream_t s);
+    void launch_composite_list(const Consts& k, const CompositeArgs& a, hipStream_t s);         \
+    void launch_composite_queue(const Consts& k, const CompositeArgs& a, hipStream_t s);
 
 // Each namespace is reopened with the inline-ness this translation unit
 // gave it in atmo_device.h.

@@ -770,15 +770,61 @@ k_composite(Consts k, CompositeArgs a) {
     store_tile_by_last_wave(tile, &waves_done, a.hdr, base, npix);
 }
 
+// The surrogate composite's queued pixels (out of the network's domain, or
+// routed to the plain integrator) through the coarse-only limb integral of
+// k_composite, one thread per queue entry at four waves per SIMD.  A
+// finished pixel is written and its entry set to -1; an entry left in place
+// needs the terminator refinement, which k_composite_list adds after this
+// kernel.  The grid covers the largest possible queue; blocks past the
+// queue's count exit at once.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ATMO_COMPOSITE_WAVES, 8)))
+k_composite_queue_coarse(Consts k, CompositeArgs a) {
+    __shared__ float4 stash[256];
+    const long count = min((long)a.queue[a.parity], a.queue_capacity);
+    const uint32_t base = blockIdx.x * 256u;
+    if ((long)base + threadIdx.x >= count) return;
+    // the wave's first thread index, wave-uniform (an SGPR)
+    const uint32_t wave_base = __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
+    {
+        const uint32_t p = (uint32_t)a.queue[2 + base + threadIdx.x];
+        const f3 ray = pixel_ray(a, p);
+        const float sd = norm(pix3(a.pos, p));
+        stash[threadIdx.x] = make_float4(ray.x, ray.y, ray.z, sd);
+    }
+    f3 scat;
+    bool done;
+    {
+        const float4 r = stash[threadIdx.x];
+        done = surface_inscatter_t<false>(k, a.tlut, a.mslut, a.cam_alt, mk(r.x, r.y, r.z), r.w,
+                                          a.sun, a.irradiance, a.bottom, a.ar, scat);
+    }
+    // as in k_composite: the entry, ray and distance come back after the
+    // interval loop (from the queue and LDS) instead of staying live in
+    // registers across it
+    asm volatile("" ::: "memory");
+    const uint32_t t = wave_base + __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    if (done) {
+        const uint32_t e = base + t;
+        const uint32_t j = (uint32_t)a.queue[2 + e];
+        const float4 r = stash[t];
+        const f3 trans = surface_transmittance(k, a.tlut, a.cam_alt, mk(r.x, r.y, r.z), r.w,
+                                               a.bottom);
+        put3(a.hdr, j, maxs(scat + pix3(a.scene, j) * trans, 0.0f));
+        a.queue[2 + e] = -1;
+    }
+}
+
 // The surface pixels deferred to the queue: the full surface branch of the
 // COMPOSITE view, refinement included (pixels the coarse pass could not
-// finish, or the surrogate composite's out-of-domain pixels).  Block 0 then
-// zeroes the other counter for the next call.
+// finish; entries that k_composite_queue_coarse finished hold -1 and are
+// skipped).  Block 0 then zeroes the other counter for the next call.
 __global__ void __launch_bounds__(256) k_composite_list(Consts k, CompositeArgs a) {
     const int count = (int)min((long)a.queue[a.parity], a.queue_capacity);
     if (blockIdx.x == 0 && threadIdx.x == 0) a.queue[1 - a.parity] = 0;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < count; e += gridDim.x * blockDim.x) {
-        const uint32_t j = (uint32_t)a.queue[2 + e];
+        const int q = a.queue[2 + e];
+        if (q < 0) continue;
+        const uint32_t j = (uint32_t)q;
         f3 ray = pixel_ray(a, j);
         float sd = norm(pix3(a.pos, j));
         f3 scat = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun, a.irradiance,
@@ -919,6 +965,11 @@ void launch_prepare(const Consts& k, long npix, const float* position_view,
 
 void launch_composite_list(const Consts& k, const CompositeArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_composite_list, dim3(256), dim3(256), 0, s, k, a);
+}
+
+void launch_composite_queue(const Consts& k, const CompositeArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_composite_queue_coarse, dim3(grid_for(a.queue_capacity)), dim3(256), 0, s,
+                       k, a);
 }
 
 void launch_composite(const Consts& k, const CompositeArgs& a, hipStream_t s) {

@@ -257,6 +257,16 @@ class AtmosphereRenderer:
             sun_angular_radius, f3(right), f3(view_up), f3(forward), tangent_half_fov,
             star_contrast_start, star_contrast_end, self._stream()), self._handle)
 
+    def composite_queued_pixels(self) -> int:
+        """Surface pixels the last COMPOSITE-view call left to the plain
+        integrator's second pass (atmo_composite_queued): refinement pixels
+        of the plain composite; out-of-domain and routed pixels of the
+        surrogate composite.  Waits for the renderer's stream."""
+        count = ctypes.c_int64(0)
+        check(self._lib.atmo_composite_queued(self._handle, ctypes.byref(count), self._stream()),
+              self._handle)
+        return int(count.value)
+
     # ------------------------------------------------------------ autotuner
 
     def autotune(self, planet_radius_m: float, camera_radius_m: float, sun_local,

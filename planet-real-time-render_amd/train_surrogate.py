@@ -37,6 +37,9 @@ def main(argv=None) -> int:
                     help="horizon-relative asinh stretch of the cosines (0: raw cosines)")
     ap.add_argument("--network-transmittance", action="store_true",
                     help="composite with the network's transmittance instead of the exact one")
+    ap.add_argument("--route-share", type=float, default=None,
+                    help="plain integrator for pixels whose learned in-scatter exceeds this "
+                         "share of the pixel (default: surrogate.ROUTE_SHARE; 0: no routing)")
     ap.add_argument("--out", type=Path, default=None,
                     help="output path without suffix (default: the shipped weights)")
     args = ap.parse_args(argv)
@@ -62,6 +65,8 @@ def main(argv=None) -> int:
                             cosine_stretch=(sur.COSINE_STRETCH if args.cosine_stretch is None
                                             else args.cosine_stretch),
                             exact_transmittance=not args.network_transmittance,
+                            route_share=(sur.ROUTE_SHARE if args.route_share is None
+                                         else args.route_share),
                             log=log, **kwargs)
     if args.out is None:
         args.out = sur.default_weights_path()

@@ -124,8 +124,11 @@ def test_init_params_layout(pkg):
 
 def test_shipped_weights_record_their_gate_and_the_facade_warns(pkg):
     """The shipped network's weight file records its validation against the
-    float64 restatement (it misses the 1 % p99 gate), and the facade's
-    composite warns once per network before using such weights."""
+    float64 restatement: with its routing (route_share) it meets the 1 % p99
+    gate, while the network alone misses it.  The facade's composite warns
+    once per network before using weights that record a missed gate, and
+    not for the shipped ones."""
+    import copy
     import importlib
     import json
     import warnings
@@ -136,14 +139,21 @@ def test_shipped_weights_record_their_gate_and_the_facade_warns(pkg):
     validation = meta["validation"]
     assert validation["gate_p99"] == 0.01
     assert validation["gate_met"] is (validation["hdr_p99_vs_fp64"] <= 0.01)
+    assert validation["gate_met"] and meta["route_share"] == validation["route_share"] > 0.0
+    assert validation["network_alone"]["hdr_p99_vs_fp64"] > 0.01
+    missed = copy.deepcopy(meta)
+    missed["validation"]["gate_met"] = False
 
-    class Net:  # the attributes the facade reads, without a device
-        metadata = meta
-        gate_met = sur.SurrogateMLP.gate_met.fget(type("M", (), {"metadata": meta})())
+    def net_of(m):  # the attributes the facade reads, without a device
+        class Net:
+            metadata = m
+            gate_met = sur.SurrogateMLP.gate_met.fget(type("M", (), {"metadata": m})())
+        return Net()
 
-    net = Net()
+    shipped, below = net_of(meta), net_of(missed)
     with warnings.catch_warnings(record=True) as caught:
         warnings.simplefilter("always")
-        renderer._warn_if_below_gate(net)
-        renderer._warn_if_below_gate(net)
+        renderer._warn_if_below_gate(shipped)
+        renderer._warn_if_below_gate(below)
+        renderer._warn_if_below_gate(below)
     assert len(caught) == 1 and "accuracy gate" in str(caught[0].message)
