@@ -759,9 +759,13 @@ static inline void segment_integral(v3 ext, real step, v3* transmission,
     for (int c = 0; c < 3; ++c) {
         out[c] = step;
 #ifdef ORACLE_ROBUST
-        /* 1 - exp(-x) cancels for small x: -expm1(-x), rounded once */
-        if (e[c] > 1.0e-12)
-            out[c] = (real)(-expm1(-(wide)fmin(e[c] * step, 80.0))) / e[c];
+        /* 1 - exp(-x) cancels for small x: there -expm1(-x), rounded once.
+           For x >= 1/16 it is at least 0.06, so float32's 1 - T carries it
+           to about 1e-6 and needs no float64. */
+        if (e[c] > 1.0e-12) {
+            const real x = fmin(e[c] * step, 80.0);
+            out[c] = x < 0.0625 ? (real)(-expm1(-(wide)x)) / e[c] : (1.0 - t[c]) / e[c];
+        }
 #else
         if (e[c] > 1.0e-12) out[c] = (1.0 - t[c]) / e[c];
 #endif

@@ -908,11 +908,14 @@ __device__ __forceinline__ float exp_neg_segment(float x) {
 
 #if ATMO_ROBUST
 // One channel of the robust segment: T = e^-x exactly as exp_neg_segment,
-// and (1 - e^-x) / sigma from 1 - e^-x in float64, rounded once.  For
-// x < 1/16 both come from one polynomial: q = (1 - e^-x) / x is the Taylor
-// sum whose last Horner step gives exp_neg_segment's e^-x = 1 - x q, so T
-// keeps its bits and 1 - e^-x = x q has no cancellation.  Above 1/16,
-// 1 - exp(-x) in float64 is within 16 float64 ulps, far below float32.
+// and (1 - e^-x) / sigma without the cancellation of 1 - T for small x.
+// For x < 1/16 both come from one polynomial: q = (1 - e^-x) / x is the
+// Taylor sum whose last Horner step gives exp_neg_segment's e^-x = 1 - x q,
+// so T keeps its bits and 1 - e^-x = x q, formed in float64 and rounded
+// once, has no cancellation.  For x >= 1/16, 1 - e^-x >= 0.06 and the
+// float32 1 - T carries it to about 1e-6, so no float64 is spent there (a
+// float64 exp measured no accuracy gain against the float64 restatement on
+// the limb-centred stress frames at any altitude).
 __device__ __forceinline__ void robust_segment(float x, float sigma, float step, float& t_out,
                                                float& i_out) {
     if (x < 0.0625f) {
@@ -931,7 +934,7 @@ __device__ __forceinline__ void robust_segment(float x, float sigma, float step,
         return;
     }
     t_out = expf(-x);
-    i_out = sigma > 1.0e-12f ? cdiv((float)(1.0 - exp(-(double)x)), sigma) : step;
+    i_out = sigma > 1.0e-12f ? cdiv(1.0f - t_out, sigma) : step;
 }
 #endif
 

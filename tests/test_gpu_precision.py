@@ -85,11 +85,12 @@ def test_robust_matches_reference_oracle_where_float32_is_well_conditioned(pkg, 
         assert worst <= PLAIN_TOLERANCE, key
 
 
-# Upper bounds of the shipped network's hdr p99 against float64 on the
-# in-domain stress frames, about 1.5x the values measured on MI355X (3.56,
-# 2.84, 8.8e-2 at 480x270): it misses the 1 % gate there by up to two orders
-# of magnitude, and a regression fails here.
-STRESS_SURROGATE_P99 = {0.5: 5.5, 2_000.0: 4.3, 100_000.0: 0.13}
+# Upper bounds of the shipped surrogate's hdr p99 against float64 on the
+# in-domain stress frames (its routing on), about 2x the values measured on
+# MI355X at 480x270: 0.187 at 0.5 m, 4.2e-3 at 2 km, 2.9e-3 at 100 km.  The
+# limb-centred view from 0.5 m, two thirds near-horizon rays, misses the 1 %
+# gate; a regression fails here.
+STRESS_SURROGATE_P99 = {0.5: 0.4, 2_000.0: 1e-2, 100_000.0: 1e-2}
 
 
 @pytest.mark.parametrize("altitude", [0.5, 2_000.0, 100_000.0, 400_000.0, 2_000_000.0])
@@ -99,8 +100,8 @@ def test_stress_surrogate_error_vs_fp64(pkg, scene, ref64, device, altitude):
     the float64 restatement reported beside the plain path's.  Cameras above
     the network's 100 km training domain fall back to the plain integrator
     for every pixel, so there the surrogate frame is the plain frame bit for
-    bit; inside the domain its error is bounded by the values measured on
-    these frames (the network misses the 1 % gate)."""
+    bit; inside the domain its error (with the shipped routing) is bounded by
+    the values measured on these frames."""
     import importlib
 
     import torch
@@ -119,11 +120,10 @@ def test_stress_surrogate_error_vs_fp64(pkg, scene, ref64, device, altitude):
     outs = {}
     for name, s in (("plain", None), ("surrogate", net)):
         hdr = torch.zeros_like(sc)
-        with pytest.warns(RuntimeWarning) if s is not None else _nullcontext():
-            r.composite(sc, sp, sid, pos, hdr, R, rc, inputs.sun_local, light.solar_irradiance,
-                        light.sun_disk_radiance, light.sun_angular_radius_degrees,
-                        inputs.view_basis, inputs.tangent_half_fov, *inputs.star_contrast,
-                        surrogate=s)
+        r.composite(sc, sp, sid, pos, hdr, R, rc, inputs.sun_local, light.solar_irradiance,
+                    light.sun_disk_radiance, light.sun_angular_radius_degrees,
+                    inputs.view_basis, inputs.tangent_half_fov, *inputs.star_contrast,
+                    surrogate=s)
         torch.cuda.synchronize(device)
         outs[name] = hdr.cpu().numpy()
     e_plain = summary(rel_error(outs["plain"], want64))
@@ -138,10 +138,3 @@ def test_stress_surrogate_error_vs_fp64(pkg, scene, ref64, device, altitude):
     r.close()
     net.close()
 
-
-class _nullcontext:
-    def __enter__(self):
-        return self
-
-    def __exit__(self, *exc):
-        return False
