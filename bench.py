@@ -39,6 +39,9 @@ import __graft_entry__ as entry  # noqa: E402
 
 METRIC = "1080p atmosphere frame ms (LUTs+composite): surrogate vs plain @ p99 rel err<=1%"
 FP32_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # dense
+# the surrogate network's algorithmic FLOP per pixel: 2 (5x64 + 64x64 + 64x64 + 64x6)
+MLP_FLOP_PER_PIXEL = 2 * (5 * 64 + 64 * 64 + 64 * 64 + 64 * 6)
 FP32_NONPACKED_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
@@ -69,7 +72,7 @@ COMPOSITE_BYTES_PER_PIXEL = 4 + 12 + 12 + 12
 # HBM bytes one 1080p k_composite launch moves, from the FETCH_SIZE and
 # WRITE_SIZE counter passes (scaled by a 256 MiB calibration copy) recorded
 # in profiles/atmosphere_frame_1080p_hbm_traffic.md.
-COMPOSITE_TRAFFIC_BYTES_1080P = 108.82e6
+COMPOSITE_TRAFFIC_BYTES_1080P = 108.85e6
 # Memory moves whole cache lines (128 B on MI355X's L2).
 CACHE_LINE_BYTES = 128
 
@@ -590,9 +593,17 @@ def main() -> int:
         plain_share = frame.r.composite_queued_pixels() / max(int((inputs.surface_id >= 0).sum()), 1)
         log(f"surrogate: route_share {net.route_share}, {plain_share:.3f} of surface pixels "
             f"on the plain path")
+        route_share = net.route_share
+        # The network kernel alone (route_share 0, so the queued passes after
+        # it are empty): the surrogate frame's dominant kernel, for its roofline.
+        net.set_route_share(0.0)
+        _, _, a_stage_ms = time_frames(sur_stages[-1:], args.steps, args.warmup, stream, device)
+        n_network = int((inputs.surface_id >= 0).sum()) - frame.r.composite_queued_pixels()
         surrogate = {"wall_ms": s_wall, "frame_ms": s_frame_ms, "stage_ms": s_stage_ms,
                      "hdr": sur_hdr, "weights": display_path(weights),
-                     "route_share": net.route_share, "plain_path_share": plain_share}
+                     "route_share": route_share, "plain_path_share": plain_share,
+                     "network_alone_ms": a_stage_ms["composite_surrogate"],
+                     "network_pixels": n_network}
         net.close()
     else:
         log(f"no surrogate weights at {weights}.bin: surrogate frame not measured")
@@ -747,6 +758,27 @@ def main() -> int:
         "hbm_gbs": round(bytes_per_launch / comp_s / 1e9, 2),
         "hbm_frac": round(bytes_per_launch / comp_s / 1e9 / HBM_PEAK_GBS, 5),
     }
+    # Roofline of the surrogate frame's dominant kernel, k_composite_surrogate
+    # timed alone: the network's algorithmic MFMA FLOP against the dense BF16
+    # peak (the same kernel also runs the sky pixels and the exact two-sample
+    # transmittance, which this count leaves out).
+    surrogate_roofline = None
+    if surrogate is not None:
+        net_s = surrogate["network_alone_ms"] * 1e-3
+        mflop = MLP_FLOP_PER_PIXEL * surrogate["network_pixels"]
+        surrogate_roofline = {
+            "bound": "mfma",
+            "achieved": round(mflop / net_s / 1e12, 3),
+            "peak": BF16_MFMA_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(mflop / net_s / 1e12 / BF16_MFMA_PEAK_TFLOPS, 4),
+            "traffic": None,
+            "kernel": ("k_composite_surrogate, network alone (route_share 0): one "
+                       "atmo_composite_surrogate call, whose two queued passes are then empty"),
+            "kernel_ms": round(surrogate["network_alone_ms"], 4),
+            "algorithmic_flop": mflop,
+            "network_pixels": surrogate["network_pixels"],
+        }
     # Roofline of every LUT build and of the surface lighting, single state
     # (the plain frame's per-stage event medians) and, per stage kernel, the
     # 256-state batch.
@@ -817,6 +849,7 @@ def main() -> int:
         "data": "synthetic",
         "config": config,
         "roofline": roofline,
+        "surrogate_roofline": surrogate_roofline,
         "lut_roofline": lut_roofline,
         "cpu_baseline": cpu,
         "accuracy": accuracy,
