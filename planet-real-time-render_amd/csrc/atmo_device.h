@@ -1010,6 +1010,49 @@ __device__ __forceinline__ void scattering_segment(const Consts& k, const float*
         integ;
 }
 
+// One base interval [d0, d1] of the bounded direct limb integral: its
+// terminator-refinement weight (_terminator_refinement_weight,
+// renderer.py:901-966) and its coarse segment (in-scatter `cl`,
+// transmittance `ct`) at the shared midpoint: the same operations as the
+// first half of an iteration of surface_inscatter_t's interval loop.
+// (surface_inscatter_t keeps its own copy: moved into a function, the
+// coarse-only k_composite spills 16 bytes per lane at 128 VGPRs.)
+__device__ __forceinline__ void limb_coarse(const Consts& k, const float* __restrict__ tlut,
+                                            const float* __restrict__ mslut, float h0, f3 ray,
+                                            f3 origin, float oc, float r0, float r1, bool v0,
+                                            bool v1, const RayPhase& ph, f3 sun, f3 irradiance,
+                                            float bottom, float ar, float d0, float d1,
+                                            float& weight, f3& cl, f3& ct) {
+    float mid = (d0 + d1) * 0.5f;
+    float width = fmaxf(d1 - d0, 1.0e-4f);
+    // with neither shadow root the proximity term is exactly 0
+    float prox = 0.0f;
+    if (v0 || v1) {
+        float closest = 1.0e30f;
+        if (v0) closest = fminf(closest, fabsf(mid - r0));
+        if (v1) closest = fminf(closest, fabsf(mid - r1));
+        prox = clampf(2.0f - cdiv(closest, width), 0.0f, 1.0f);
+        prox = prox * prox * (3.0f - 2.0f * prox);
+    }
+    f3 point = origin + ray * mid;
+    float radius = fmaxf(norm(point), 1.0f);
+    f3 radial = div3(point, radius);
+    float alt = altitude_on_ray(h0, oc, mid, bottom);
+    float sun_cosine = dot(radial, sun);
+    SunGeom g = sun_geometry(k, alt, sun_cosine, bottom, ar);
+    weight = fmaxf(prox, 4.0f * g.visibility * (1.0f - g.visibility));
+    float step = fmaxf(d1 - d0, 0.0f);
+    f3 d = density(k, alt);
+    f3 integ;
+    segment_integral(extinction_of(k, d), step, ct, integ);
+    f3 st = solar_transmittance(k, tlut, g, alt, bottom, ar);
+    f3 phase_term = (arr3(k.beta_r) * d.x) * ph.pr + (arr3(k.beta_ms) * d.y) * ph.pm;
+    f3 src = (irradiance * st) * phase_term;
+    f3 ms = sample_multi_scattering(k, mslut, alt, sun_cosine, bottom);
+    src = src + scattering_of(k, d) * ms;
+    cl = src * integ;
+}
+
 // The bounded direct limb integral of a surface pixel
 // (_integrate_surface_scattering, renderer.py:1667-1755, with the origin of
 // _surface_aerial_scattering, renderer.py:1757-1786): base intervals with
@@ -1110,6 +1153,118 @@ __device__ __forceinline__ bool surface_inscatter_t(const Consts& k, const float
     }
     out = maxs(radiance, 0.0f);
     return true;
+}
+
+// surface_inscatter_t<true> spread over a group of G lanes (G >= hz_steps
+// and G >= term_sub, a power of two; `sub` is the lane's index in the group,
+// `stage_a` / `stage_b` the group's G slots of LDS).  Lane i takes base
+// interval i's coarse segment and refinement weight.  The refined intervals'
+// term_sub sub-segments each are then dealt out over all G lanes, G per
+// round, and each refined interval's lane folds its own sub-segments from
+// LDS in sub-segment order, exactly as the sequential loop does.  Last, the
+// group folds the intervals' (in-scatter, transmittance) pairs in interval
+// order by a butterfly, each step putting the lower half's pair before the
+// upper half's.  The value equals surface_inscatter's up to the association
+// of that last fold; every lane of the group returns it.  The pixel's
+// arguments are group-uniform and the group's lanes are all active.
+template <int G>
+__device__ __forceinline__ f3 surface_inscatter_lanes(const Consts& k, const float* __restrict__ tlut,
+                                                      const float* __restrict__ mslut, float h0,
+                                                      f3 ray, float surface_distance, f3 sun,
+                                                      f3 irradiance, float bottom, float ar,
+                                                      int sub, float4* stage_a, float2* stage_b) {
+    float on, of;
+    bool oh;
+    shell_roots(k, h0, ray.y, k.top, bottom, on, of, oh);
+    float start = fmaxf(on, 0.0f);
+    float end = fminf(surface_distance, of);
+    if (!(oh && end > start)) return mk(0.0f, 0.0f, 0.0f);  // group-uniform
+
+    const f3 origin = mk(0.0f, bottom + h0, 0.0f);
+    float r0, r1;
+    bool v0, v1;
+    solar_shadow_roots(origin, ray, sun, bottom, r0, r1, v0, v1);
+    const Warp w = make_warp(-dot(origin, ray), start, end);
+    const float origin_radius = fmaxf(norm(origin), 1.0f);
+    const float oc = dot(div3(origin, origin_radius), ray);
+    const RayPhase ph = ray_phase(k, ray, sun);
+    const int n = k.hz_steps;
+    float d0 = 0.0f, d1 = 0.0f, weight = 0.0f;
+    f3 cl = mk(0.0f, 0.0f, 0.0f), ct = mk(1.0f, 1.0f, 1.0f);
+    if (sub < n) {
+        const float fn = (float)n, rn = 1.0f / fn;
+        // the same boundaries as the sequential loop's
+        d0 = sub == 0 ? boundary(w, 0.0f / fn) : boundary(w, quot((float)sub, fn, rn));
+        d1 = boundary(w, quot((float)(sub + 1), fn, rn));
+        limb_coarse(k, tlut, mslut, h0, ray, origin, oc, r0, r1, v0, v1, ph, sun, irradiance,
+                    bottom, ar, d0, d1, weight, cl, ct);
+    }
+    const bool refined = sub < n && weight > 0.0f;
+    const int first_lane = (int)(threadIdx.x % 64) & ~(G - 1);
+    const uint64_t ballot = __ballot(refined);
+    const uint64_t mask = G == 64 ? ballot : (ballot >> first_lane) & ((1ull << (G & 63)) - 1ull);
+    f3 fl = mk(0.0f, 0.0f, 0.0f), ft = mk(1.0f, 1.0f, 1.0f);
+    if (mask != 0) {  // group-uniform
+        const int m = k.term_sub;
+        const int items = __popcll(mask) * m;
+        // this lane's sub-segments are items [first, first + m) when it is refined
+        const int first = __popcll(mask & ((1ull << sub) - 1ull)) * m;
+        for (int round = 0; round < items; round += G) {
+            const int t = min(round + sub, items - 1);
+            const int kth = t / m, j = t - kth * m;
+            uint64_t mm = mask;
+            for (int q = 0; q < kth; ++q) mm &= mm - 1ull;
+            const int iv = __ffsll((unsigned long long)mm) - 1;
+            const float a0 = __shfl(d0, iv, G), a1 = __shfl(d1, iv, G);
+            f3 sl = mk(0.0f, 0.0f, 0.0f), stt = mk(1.0f, 1.0f, 1.0f);
+            if (round + sub < items) {
+                // the sequential loop's sub-segment bounds
+                float f0 = (float)j / (float)m;
+                float f1 = (float)(j + 1) / (float)m;
+                float cs = a0 + (a1 - a0) * f0;
+                float ce = a0 + (a1 - a0) * f1;
+                scattering_segment(k, tlut, mslut, origin, h0, oc, ray, cs, ce, ph, sun,
+                                   irradiance, bottom, ar, sl, stt);
+            }
+            stage_a[sub] = make_float4(sl.x, sl.y, sl.z, stt.x);
+            stage_b[sub] = make_float2(stt.y, stt.z);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (refined) {
+                const int lo = max(first, round), hi = min(first + m, round + G);
+                for (int q = lo; q < hi; ++q) {
+                    const float4 va = stage_a[q - round];
+                    const float2 vb = stage_b[q - round];
+                    fl = fl + ft * mk(va.x, va.y, va.z);
+                    ft = ft * mk(va.w, vb.x, vb.y);
+                }
+            }
+            // every lane has read this round's slots before the next round writes them
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    f3 il = cl, it = ct;
+    if (refined) {
+        il = cl * (1.0f - weight) + fl * weight;
+        it = ct * (1.0f - weight) + ft * weight;
+    }
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+        const f3 pl = mk(__shfl_xor(il.x, off, G), __shfl_xor(il.y, off, G),
+                         __shfl_xor(il.z, off, G));
+        const f3 pt = mk(__shfl_xor(it.x, off, G), __shfl_xor(it.y, off, G),
+                         __shfl_xor(it.z, off, G));
+        // one expression for both lanes of a pair, so both hold the same bits
+        const bool upper = (sub & off) != 0;
+        const f3 lo_l = upper ? pl : il, lo_t = upper ? pt : it;
+        const f3 hi_l = upper ? il : pl, hi_t = upper ? it : pt;
+        il = lo_l + lo_t * hi_l;
+        it = lo_t * hi_t;
+    }
+    return maxs(il, 0.0f);
 }
 
 __device__ __forceinline__ f3 surface_inscatter(const Consts& k, const float* __restrict__ tlut,

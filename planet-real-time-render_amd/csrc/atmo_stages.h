@@ -817,20 +817,41 @@ k_composite_queue_coarse(Consts k, CompositeArgs a) {
 // The surface pixels deferred to the queue: the full surface branch of the
 // COMPOSITE view, refinement included (pixels the coarse pass could not
 // finish; entries that k_composite_queue_coarse finished hold -1 and are
-// skipped).  Block 0 then zeroes the other counter for the next call.
-__global__ void __launch_bounds__(256) k_composite_list(Consts k, CompositeArgs a) {
+// skipped).  G lanes per entry (surface_inscatter_lanes): one base interval
+// per lane, then the refined intervals' sub-segments dealt out over the
+// group, so a pixel costs one coarse segment and a few rounds of
+// sub-segments per lane instead of every segment in one thread, and a queue
+// of a few thousand pixels still fills the GPU.
+// Block 0 then zeroes the other counter for the next call.
+// Three waves per SIMD (168 VGPRs, 64 bytes of spill): on a twilight
+// 960x540 frame with 40 K refined pixels the composite takes 176 us, against
+// 191 us at two waves (189 VGPRs) and 219 us at four (a 220-byte spill).
+#ifndef ATMO_LIST_WAVES
+#define ATMO_LIST_WAVES 3
+#endif
+template <int G>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ATMO_LIST_WAVES, 8)))
+k_composite_list(Consts k, CompositeArgs a) {
+    __shared__ float4 stage_a[256];
+    __shared__ float2 stage_b[256];
     const int count = (int)min((long)a.queue[a.parity], a.queue_capacity);
     if (blockIdx.x == 0 && threadIdx.x == 0) a.queue[1 - a.parity] = 0;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < count; e += gridDim.x * blockDim.x) {
+    const int sub = (int)threadIdx.x % G;
+    const int group_first = (int)threadIdx.x - sub;
+    const long stride = (long)gridDim.x * (blockDim.x / G);
+    for (long e = ((long)blockIdx.x * blockDim.x + threadIdx.x) / G; e < count; e += stride) {
         const int q = a.queue[2 + e];
-        if (q < 0) continue;
+        if (q < 0) continue;  // group-uniform
         const uint32_t j = (uint32_t)q;
         f3 ray = pixel_ray(a, j);
         float sd = norm(pix3(a.pos, j));
-        f3 scat = surface_inscatter(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun, a.irradiance,
-                                    a.bottom, a.ar);
-        f3 trans = surface_transmittance(k, a.tlut, a.cam_alt, ray, sd, a.bottom);
-        put3(a.hdr, j, maxs(scat + pix3(a.scene, j) * trans, 0.0f));
+        f3 scat = surface_inscatter_lanes<G>(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun,
+                                             a.irradiance, a.bottom, a.ar, sub,
+                                             stage_a + group_first, stage_b + group_first);
+        if (sub == 0) {
+            f3 trans = surface_transmittance(k, a.tlut, a.cam_alt, ray, sd, a.bottom);
+            put3(a.hdr, j, maxs(scat + pix3(a.scene, j) * trans, 0.0f));
+        }
     }
 }
 
@@ -963,8 +984,25 @@ void launch_prepare(const Consts& k, long npix, const float* position_view,
                        sky_l);
 }
 
+#ifndef ATMO_LIST_BLOCKS
+#define ATMO_LIST_BLOCKS 2048
+#endif
+template <int G>
+void launch_composite_list_g(const Consts& k, const CompositeArgs& a, hipStream_t s) {
+    // one group per entry up to ATMO_LIST_BLOCKS blocks, then a grid-stride loop
+    long blocks = (a.queue_capacity * G + 255) / 256;
+    if (blocks > ATMO_LIST_BLOCKS) blocks = ATMO_LIST_BLOCKS;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_composite_list<G>, dim3((unsigned)blocks), dim3(256), 0, s, k, a);
+}
+
 void launch_composite_list(const Consts& k, const CompositeArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_composite_list, dim3(256), dim3(256), 0, s, k, a);
+    // the smallest power-of-two group that holds hz_steps (4..64) intervals
+    if (k.hz_steps <= 4) launch_composite_list_g<4>(k, a, s);
+    else if (k.hz_steps <= 8) launch_composite_list_g<8>(k, a, s);
+    else if (k.hz_steps <= 16) launch_composite_list_g<16>(k, a, s);
+    else if (k.hz_steps <= 32) launch_composite_list_g<32>(k, a, s);
+    else launch_composite_list_g<64>(k, a, s);
 }
 
 void launch_composite_queue(const Consts& k, const CompositeArgs& a, hipStream_t s) {

@@ -137,6 +137,28 @@ def test_terminator_frames_parity(pkg, scene, ref32, device):
             check(f"terminator {k} {name}", got[name], want[name])
 
 
+@pytest.mark.parametrize("steps, substeps", [(4, 2), (7, 16), (12, 5), (33, 3), (64, 16)])
+def test_terminator_lane_groups_parity(pkg, scene, ref32, device, steps, substeps):
+    """The refined pixels' integrator runs one lane per base interval in
+    groups of 4, 8, 16, 32 or 64 lanes and deals the refined intervals'
+    sub-segments out over the group in rounds; these interval and
+    sub-segment counts take every group size, sub-segment counts that do not
+    divide the group, and (7, 16) an interval whose sub-segments span two
+    rounds."""
+    inputs = scene.make_frame(width=160, height=90, config_index=1, seed_offset=10)
+    inputs.sun_local = scene.sun_from_angles(90.5, 0.3)
+    inputs.lighting = pkg.LightingState(inputs.sun_local, 0.266, scene.SOLAR_IRRADIANCE)
+    inputs.camera_radius_m = inputs.planet_radius_m + 20_000.0
+    inputs.config = inputs.config.replace(aerial_horizon_raymarch_steps=steps,
+                                          aerial_terminator_substeps=substeps)
+    got = gpu_frame(pkg, inputs, device)
+    want = ref32.frame(inputs)
+    print(f"{steps} intervals x {substeps} sub-segments: refined intervals "
+          f"{want['refined_intervals']}", flush=True)
+    assert want["refined_intervals"] > 0
+    check(f"lane groups {steps}x{substeps} hdr", got["hdr"], want["hdr"])
+
+
 @pytest.mark.parametrize("width, height", [(1, 1), (1, 7), (5, 3), (17, 13), (255, 1), (1, 257),
                                            (16, 17)])
 def test_tiny_and_ragged_frames_parity(pkg, scene, ref32, device, width, height):
