@@ -26,8 +26,11 @@ struct AtmoHandle {
     float* sky_l;
     bool borrowed[6];  // buffer supplied by the caller through atmo_bind_buffer
     // Pixels deferred by the COMPOSITE passes (refinement, surrogate
-    // fallback): [0], [1] = two counters, then pixel indices.  A call
-    // counts in [parity]; its last pass zeroes [1 - parity] for the next.
+    // fallback): [0], [1] = two counters, then width x height pixel indices.
+    // A call counts in [parity]; its last pass zeroes [1 - parity] for the
+    // next.  The same allocation holds a second queue of that layout right
+    // after it (refine_queue()): the surrogate fallback's pixels that need
+    // the terminator refinement.
     int* queue;
     int queue_parity;
     bool queue_reset;  // a composite launch failed: clear both counters first
@@ -59,6 +62,12 @@ struct CompositeArgs {
     int* queue;  // COMPOSITE view: deferred pixels, layout of AtmoHandle::queue
     int parity;  // the counter of this call: queue[parity]
     long queue_capacity;  // index slots after the two counters
+    // a second queue of the same layout and parity for the pixels the
+    // coarse pass over `queue` leaves to the refinement, or null
+    int* refine = nullptr;
+    // the queue the list pass (k_composite_list) works through: `queue` by
+    // default, `refine` after the coarse pass over `queue`
+    int* list = nullptr;
     // frame constants (composite_frame_constants)
     float aspect;        // width / height
     atmo::f3 fwd_t;      // forward / thf
@@ -89,23 +98,28 @@ struct BatchState {
 };
 
 // Launch the full surface branch of the COMPOSITE view for the pixels
-// deferred to a.queue, in the handle's precision variant (atmo_kernels.hip).
+// deferred to a.list (a.queue when null), in the handle's precision variant
+// (atmo_kernels.hip).
 int32_t atmo_launch_composite_list(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream);
 
 // Launch the coarse-only surface branch for every pixel of a.queue (the
-// surrogate composite's fallback): finished pixels are written and their
-// entries set to -1, the rest are left for atmo_launch_composite_list
-// (atmo_kernels.hip).
+// surrogate composite's fallback): finished pixels are written, the rest are
+// appended to a.refine for atmo_launch_composite_list (atmo_kernels.hip).
 int32_t atmo_launch_composite_queue(AtmoHandle* h, const CompositeArgs& a, hipStream_t stream);
 
 // Clears both counters of a deferred-pixel queue with a one-wave kernel
 // (no memset node, so the launch can sit in a captured graph).
 void atmo_launch_clear_queue(int* queue, hipStream_t stream);
 
-// Prepares the handle's queue for one COMPOSITE-view call: clears the
-// counters if an earlier call failed, and fills a.queue, a.parity and
-// a.queue_capacity (atmo_kernels.hip).
+// Prepares the handle's queues for one COMPOSITE-view call: clears the
+// counters if an earlier call failed, and fills a.queue, a.refine, a.parity
+// and a.queue_capacity (atmo_kernels.hip).
 int32_t atmo_begin_queue(AtmoHandle* h, CompositeArgs& a, hipStream_t stream);
+
+// The handle's second queue (AtmoHandle::queue).
+inline int* refine_queue(const AtmoHandle* h) {
+    return h->queue + 2 + (long)h->width * h->height;
+}
 
 inline atmo::f3 mk3(const float* a) { return atmo::f3{a[0], a[1], a[2]}; }
 

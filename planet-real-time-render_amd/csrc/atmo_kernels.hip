@@ -323,8 +323,12 @@ int32_t atmo_create(const AtmoParams* params, int32_t width, int32_t height, Atm
         }
     }
     h->queue = nullptr;
-    if (hipMalloc((void**)&h->queue, ((size_t)width * (size_t)height + 2) * sizeof(int)) != hipSuccess ||
-        hipMemset(h->queue, 0, 2 * sizeof(int)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    // the deferred-pixel queue and the refinement queue after it
+    if (hipMalloc((void**)&h->queue, 2 * ((size_t)width * (size_t)height + 2) * sizeof(int)) !=
+            hipSuccess ||
+        hipMemset(h->queue, 0, 2 * sizeof(int)) != hipSuccess ||
+        hipMemset(refine_queue(h), 0, 2 * sizeof(int)) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
         set_error(nullptr, "atmo_create: device allocation failed");
         free_buffers(h);
         delete h;
@@ -509,12 +513,14 @@ void atmo_launch_clear_queue(int* queue, hipStream_t stream) {
 int32_t atmo_begin_queue(AtmoHandle* h, CompositeArgs& a, hipStream_t stream) {
     if (h->queue_reset) {
         atmo_launch_clear_queue(h->queue, stream);
+        atmo_launch_clear_queue(refine_queue(h), stream);
         if (atmo_launch_status(h, "atmo_composite: clearing the pixel queue") != ATMO_OK)
             return ATMO_EHIP;
         h->queue_parity = 0;
         h->queue_reset = false;
     }
     a.queue = h->queue;
+    a.refine = refine_queue(h);
     a.parity = h->queue_parity;
     a.queue_capacity = (long)h->width * h->height;
     h->queue_parity ^= 1;

@@ -773,10 +773,12 @@ k_composite(Consts k, CompositeArgs a) {
 // The surrogate composite's queued pixels (out of the network's domain, or
 // routed to the plain integrator) through the coarse-only limb integral of
 // k_composite, one thread per queue entry at four waves per SIMD.  A
-// finished pixel is written and its entry set to -1; an entry left in place
-// needs the terminator refinement, which k_composite_list adds after this
-// kernel.  The grid covers the largest possible queue; blocks past the
-// queue's count exit at once.
+// finished pixel is written; a pixel that needs the terminator refinement
+// is appended to the refinement queue `a.refine` (one atomic per wave that
+// has any), which k_composite_list works through after this kernel.  No
+// block-wide barrier: a wave exits as soon as its pixels are done.  The
+// grid covers the largest possible queue; blocks past the queue's count
+// exit at once.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ATMO_COMPOSITE_WAVES, 8)))
 k_composite_queue_coarse(Consts k, CompositeArgs a) {
     __shared__ float4 stash[256];
@@ -803,26 +805,24 @@ k_composite_queue_coarse(Consts k, CompositeArgs a) {
     // registers across it
     asm volatile("" ::: "memory");
     const uint32_t t = wave_base + __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    const uint32_t j = (uint32_t)a.queue[2 + base + t];
     if (done) {
-        const uint32_t e = base + t;
-        const uint32_t j = (uint32_t)a.queue[2 + e];
         const float4 r = stash[t];
         const f3 trans = surface_transmittance(k, a.tlut, a.cam_alt, mk(r.x, r.y, r.z), r.w,
                                                a.bottom);
         put3(a.hdr, j, maxs(scat + pix3(a.scene, j) * trans, 0.0f));
-        a.queue[2 + e] = -1;
     }
+    enqueue_wave(!done, j, a.refine, a.parity, a.queue_capacity);
 }
 
-// The surface pixels deferred to the queue: the full surface branch of the
-// COMPOSITE view, refinement included (pixels the coarse pass could not
-// finish; entries that k_composite_queue_coarse finished hold -1 and are
-// skipped).  G lanes per entry (surface_inscatter_lanes): one base interval
+// The surface pixels deferred to a.list (a.queue when null): the full
+// surface branch of the COMPOSITE view, refinement included (the pixels
+// k_composite or k_composite_queue_coarse could not finish).  G lanes per entry (surface_inscatter_lanes): one base interval
 // per lane, then the refined intervals' sub-segments dealt out over the
 // group, so a pixel costs one coarse segment and a few rounds of
 // sub-segments per lane instead of every segment in one thread, and a queue
 // of a few thousand pixels still fills the GPU.
-// Block 0 then zeroes the other counter for the next call.
+// Block 0 also zeroes the other counter of both queues for the next call.
 // Three waves per SIMD (168 VGPRs, 64 bytes of spill): on a twilight
 // 960x540 frame with 40 K refined pixels the composite takes 176 us, against
 // 191 us at two waves (189 VGPRs) and 219 us at four (a 220-byte spill).
@@ -834,15 +834,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ATMO_L
 k_composite_list(Consts k, CompositeArgs a) {
     __shared__ float4 stage_a[256];
     __shared__ float2 stage_b[256];
-    const int count = (int)min((long)a.queue[a.parity], a.queue_capacity);
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.queue[1 - a.parity] = 0;
+    const int* const list = a.list ? a.list : a.queue;
+    const int count = (int)min((long)list[a.parity], a.queue_capacity);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.queue[1 - a.parity] = 0;
+        if (a.refine) a.refine[1 - a.parity] = 0;
+    }
     const int sub = (int)threadIdx.x % G;
     const int group_first = (int)threadIdx.x - sub;
     const long stride = (long)gridDim.x * (blockDim.x / G);
     for (long e = ((long)blockIdx.x * blockDim.x + threadIdx.x) / G; e < count; e += stride) {
-        const int q = a.queue[2 + e];
-        if (q < 0) continue;  // group-uniform
-        const uint32_t j = (uint32_t)q;
+        const uint32_t j = (uint32_t)list[2 + e];
         f3 ray = pixel_ray(a, j);
         float sd = norm(pix3(a.pos, j));
         f3 scat = surface_inscatter_lanes<G>(k, a.tlut, a.mslut, a.cam_alt, ray, sd, a.sun,

@@ -439,14 +439,26 @@ __device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x4 lo, bf16x4 hi, floatx4 
                                                     __builtin_bit_cast(bf16x8v, b), c, 0, 0, 0);
 }
 
-// SiLU z / (1 + e^-z) at hardware rate (v_exp_f32, v_rcp_f32), then
-// packed to bf16 as the next layer's B operand.
+// The MFMA inference runs every layer scaled by kSiluScale = -log2(e): with
+// layer 1's weights and every bias multiplied by it, each layer's
+// accumulator holds y = -log2(e) z, and SiLU in that scale,
+// y / (1 + 2^y) = -log2(e) silu(z), is the next layer's input unchanged:
+// the factor carries through the unscaled bf16 weights of layers 2-4, whose
+// outputs come out scaled by it too and are multiplied back by
+// kSiluUnscale = 1 / kSiluScale.  This saves the multiply of every SiLU's
+// exponent.
+constexpr float kSiluScale = -1.44269504f;
+constexpr float kSiluUnscale = -0.693147182f;
+
+// SiLU of a pre-activation scaled by kSiluScale, y / (1 + 2^y), at
+// hardware rate (v_exp_f32, v_rcp_f32), then packed to bf16 as the next
+// layer's B operand.
 __device__ __forceinline__ bf16x4 silu_bf16(floatx4 c) {
     floatx4 r;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        float z = c[i];
-        r[i] = z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504f * z));
+        float y = c[i];
+        r[i] = y * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(y));
     }
     return pack_bf16(r);
 }
@@ -612,16 +624,12 @@ __device__ __forceinline__ void mlp_wave_folded(float x1, float x3, float x4,
 // T = clamp(v, 0, 1).  Mode 0 (log): L = exp(v) - eps_inscatter,
 // T = exp(v) - eps_transmittance.  Mode 2 (floored log): L = exp(v), zero
 // below 1.5 eps_inscatter (the training floor), T as in mode 0.
-__device__ __forceinline__ void decode(const AtmoSurrogateTransform& t, const float raw[6], f3& L,
-                                       f3& T) {
-    float y[6];
+__device__ __forceinline__ f3 decode_inscatter(const AtmoSurrogateTransform& t,
+                                               const float raw[6]) {
+    float y[3];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) y[q] = raw[q] * t.out_scale[q] + t.out_mean[q];
-    if (t.output_mode == 1) {
-        L = maxs(mk(y[0], y[1], y[2]), 0.0f);
-        T = clamp3(mk(y[3], y[4], y[5]), 0.0f, 1.0f);
-        return;
-    }
+    for (int q = 0; q < 3; ++q) y[q] = raw[q] * t.out_scale[q] + t.out_mean[q];
+    if (t.output_mode == 1) return maxs(mk(y[0], y[1], y[2]), 0.0f);
     float l[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -629,9 +637,21 @@ __device__ __forceinline__ void decode(const AtmoSurrogateTransform& t, const fl
         l[q] = t.output_mode == 2 ? (e < 1.5f * t.eps_inscatter ? 0.0f : e)
                                   : fmaxf(e - t.eps_inscatter, 0.0f);
     }
-    L = mk(l[0], l[1], l[2]);
-    T = clamp3(mk(expf(y[3]) - t.eps_transmittance, expf(y[4]) - t.eps_transmittance,
-                  expf(y[5]) - t.eps_transmittance),
+    return mk(l[0], l[1], l[2]);
+}
+
+__device__ __forceinline__ void decode(const AtmoSurrogateTransform& t, const float raw[6], f3& L,
+                                       f3& T) {
+    L = decode_inscatter(t, raw);
+    float y[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) y[q] = raw[3 + q] * t.out_scale[3 + q] + t.out_mean[3 + q];
+    if (t.output_mode == 1) {
+        T = clamp3(mk(y[0], y[1], y[2]), 0.0f, 1.0f);
+        return;
+    }
+    T = clamp3(mk(expf(y[0]) - t.eps_transmittance, expf(y[1]) - t.eps_transmittance,
+                  expf(y[2]) - t.eps_transmittance),
                0.0f, 1.0f);
 }
 
@@ -674,7 +694,8 @@ struct SurrogateArgs {
 // Out-of-domain surface pixels, and in-domain ones whose learned in-scatter
 // exceeds the transform's route_share of the pixel in some channel, are
 // queued for the plain integrator (k_composite_queue_coarse, then
-// k_composite_list for those that need the terminator refinement), which
+// k_composite_list over the refinement queue for those that need the
+// terminator refinement), which
 // keeps the integrator's registers out of this kernel.  They are collected
 // in a block-local LDS queue and moved to the global queue with one atomic
 // per block flush: one global atomic per wave, all on one counter, cost
@@ -709,19 +730,19 @@ __global__ void __launch_bounds__(256) ATMO_SURROGATE_WAVES k_composite_surrogat
     if (threadIdx.x == 0) block_queue[0] = 0;
     const bf16x8* fg = reinterpret_cast<const bf16x8*>(a.frag);
     for (int e = threadIdx.x; e < kFragments * 64; e += blockDim.x) frag_s[e] = fg[e];
-    for (int e = threadIdx.x; e < 64 * 3 + 16; e += blockDim.x) bias_s[e] = a.bias[e];
+    for (int e = threadIdx.x; e < 64 * 3 + 16; e += blockDim.x) bias_s[e] = kSiluScale * a.bias[e];
     const Encoding enc = encoding_at(a.t, a.cam_alt);
     if (threadIdx.x < 64) {
         // z = W x + b with x0 (altitude) and x2 (sun cosine) fixed per frame
         const float x2 = feature_cosine(a.t, enc, clampf(a.sun.y, -1.0f, 1.0f));
         const float* w = a.params + kLayers[0].w + threadIdx.x * 5;
-        bias0_s[threadIdx.x] = (a.bias[threadIdx.x] + w[0] * enc.x0) + w[2] * x2;
+        bias0_s[threadIdx.x] = kSiluScale * ((a.bias[threadIdx.x] + w[0] * enc.x0) + w[2] * x2);
     }
     // (the first compaction's barriers order this staging before any read)
     const int lane = threadIdx.x % 64;
     float a32f[4];
 #pragma unroll
-    for (int f = 0; f < 4; ++f) a32f[f] = a.frag32[(kFolded32 + f) * 64 + lane];
+    for (int f = 0; f < 4; ++f) a32f[f] = kSiluScale * a.frag32[(kFolded32 + f) * 64 + lane];
 
     // pixel byte offsets in 32 bits: a frame holds at most 2^28 pixels
     const uint32_t npix = (uint32_t)a.width * (uint32_t)a.height;
@@ -764,10 +785,15 @@ __global__ void __launch_bounds__(256) ATMO_SURROGATE_WAVES k_composite_surrogat
                             feature_distance(enc, fmaxf(sd, a.t.distance_min_m)), frag_s, a32f,
                             bias0_s, bias_s, raw);
             if (learned) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) raw[q] *= kSiluUnscale;
                 f3 L, T;
-                decode(a.t, raw, L, T);
-                if (a.t.exact_transmittance)
+                if (a.t.exact_transmittance) {
+                    L = decode_inscatter(a.t, raw);
                     T = surface_transmittance(k, a.tlut, a.cam_alt, ray, sd, a.bottom);
+                } else {
+                    decode(a.t, raw, L, T);
+                }
                 const f3 le = L * a.irradiance;
                 const f3 st = pix3(a.scene, j) * T;
                 // the learned in-scatter's share of the pixel, per channel,
@@ -797,11 +823,11 @@ __global__ void __launch_bounds__(256) k_mlp_infer(long n, const float* __restri
     __shared__ float bias_s[64 * 3 + 16];
     const bf16x8* fg = reinterpret_cast<const bf16x8*>(frag);
     for (int e = threadIdx.x; e < kFragments * 64; e += blockDim.x) frag_s[e] = fg[e];
-    for (int e = threadIdx.x; e < 64 * 3 + 16; e += blockDim.x) bias_s[e] = bias[e];
+    for (int e = threadIdx.x; e < 64 * 3 + 16; e += blockDim.x) bias_s[e] = kSiluScale * bias[e];
     __syncthreads();
     const int lane = threadIdx.x % 64;
     float a32[8];
-    for (int f = 0; f < 8; ++f) a32[f] = frag32[f * 64 + lane];
+    for (int f = 0; f < 8; ++f) a32[f] = kSiluScale * frag32[f * 64 + lane];
     const long i_raw = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long i = i_raw < n ? i_raw : n - 1;
     float f[5];
@@ -809,7 +835,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(long n, const float* __restri
     float out[6];
     mlp_wave(f, frag_s, a32, bias_s, out);
     if (i_raw < n)
-        for (int q = 0; q < 6; ++q) y[i * 6 + q] = out[q];
+        for (int q = 0; q < 6; ++q) y[i * 6 + q] = kSiluUnscale * out[q];
 }
 
 unsigned blocks_for(long threads, int block) { return (unsigned)((threads + block - 1) / block); }
@@ -1141,6 +1167,7 @@ int32_t atmo_composite_surrogate(AtmoHandle* h, AtmoMlp* m, const float* scene_h
                        dim3(256), 0, s, h->k, a, h->queue, ca.parity, ca.queue_capacity);
     if (atmo_launch_status(h, "atmo_composite_surrogate") != ATMO_OK) return ATMO_EHIP;
     if (atmo_launch_composite_queue(h, ca, s) != ATMO_OK) return ATMO_EHIP;
+    ca.list = ca.refine;  // the queued pixels the coarse pass left to the refinement
     if (atmo_launch_composite_list(h, ca, s) != ATMO_OK) return ATMO_EHIP;
     h->queue_reset = false;
     return ATMO_OK;
