@@ -72,7 +72,7 @@ COMPOSITE_BYTES_PER_PIXEL = 4 + 12 + 12 + 12
 # HBM bytes one 1080p k_composite launch moves, from the FETCH_SIZE and
 # WRITE_SIZE counter passes (scaled by a 256 MiB calibration copy) recorded
 # in profiles/atmosphere_frame_1080p_hbm_traffic.md.
-COMPOSITE_TRAFFIC_BYTES_1080P = 108.85e6
+COMPOSITE_TRAFFIC_BYTES_1080P = 108.83e6
 # Memory moves whole cache lines (128 B on MI355X's L2).
 CACHE_LINE_BYTES = 128
 
