@@ -1155,13 +1155,14 @@ __device__ __forceinline__ bool surface_inscatter_t(const Consts& k, const float
     return true;
 }
 
-// surface_inscatter_t<true> spread over a group of G lanes (G >= hz_steps
-// and G >= term_sub, a power of two; `sub` is the lane's index in the group,
-// `stage_a` / `stage_b` the group's G slots of LDS).  Lane i takes base
-// interval i's coarse segment and refinement weight.  The refined intervals'
-// term_sub sub-segments each are then dealt out over all G lanes, G per
-// round, and each refined interval's lane folds its own sub-segments from
-// LDS in sub-segment order, exactly as the sequential loop does.  Last, the
+// surface_inscatter_t<true> spread over a group of G lanes (G >= hz_steps,
+// a power of two; `sub` is the lane's index in the group, `stage_a` /
+// `stage_b` the group's G slots of LDS).  Lane i takes base interval i's
+// coarse segment and refinement weight.  The refined intervals' term_sub
+// sub-segments each are then dealt out over all G lanes, G per round (an
+// interval's sub-segments may span two rounds), and each refined interval's
+// lane folds its own sub-segments from LDS in sub-segment order, across the
+// rounds, exactly as the sequential loop does.  Last, the
 // group folds the intervals' (in-scatter, transmittance) pairs in interval
 // order by a butterfly, each step putting the lower half's pair before the
 // upper half's.  The value equals surface_inscatter's up to the association
